@@ -1,0 +1,278 @@
+"""Benchmark: block-sparse contraction GFLOP/s on one MI355X.
+
+Default workload: C[i,j,m] = sum_{k,l} A[i,j,k,l] B[k,l,m], every mode 600
+long, split into uneven blocks drawn from {5,7,9,11,13,15,21,28} (seeded),
+blocks present with p = 0.1, values u(-1,1) * exp(-0.3*|b0-b1|), FP64.  A
+step is one execution of the contraction plan (all block products of the
+contraction, one grouped launch per kernel class) with A, B resident in HBM.
+
+Prints one JSON line.  Beside the FP64 drop-in (`value`) it reports the other
+precision forms, p50/p99 step latency, the dominant kernel's roofline from HIP
+events, parity against the CPU oracle on a sampled set of C blocks, and the
+CPU baseline (the oracle's AVX2 fma-chain restatement with a work-stealing
+pool) timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp64_exact": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+HBM_PEAK = 8.0e12
+CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
+
+CONFIGS = {
+    # name: (expr, length, block sizes or None (uneven), p, lambda, seed)
+    "c0": ("ijk,kl->ijl", 256, [16], 0.30, 0.0, 1000),
+    "c1": ("ijkl,klm->ijm", 600, None, 0.10, 0.3, 1001),
+    "c2": ("ijk,km->ijm", 1200, None, 0.10, 0.3, 1002),
+    "c3_32": ("ik,kj->ij", 4096, [32], 0.25, 0.1, 1003),
+    "c3_64": ("ik,kj->ij", 4096, [64], 0.25, 0.1, 1003),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def axis_sizes(cfg, length=None):
+    expr, L, fixed, p, lam, seed = CONFIGS[cfg]
+    L = length or L
+    letters = []
+    for ch in expr.replace("->", ",").replace(",", ""):
+        if ch not in letters:
+            letters.append(ch)
+    out = {}
+    for ax, ch in enumerate(letters):
+        if fixed:
+            b = fixed[0]
+            out[ch] = [b] * (L // b) + ([L % b] if L % b else [])
+        else:
+            import shtensor_amd as sht
+            out[ch] = sht.gen_block_sizes(seed, ax, L, CHOICES)
+    return out
+
+
+def build_tensors(sht, ctx, cfg, length=None):
+    expr, L, fixed, p, lam, seed = CONFIGS[cfg]
+    sizes = axis_sizes(cfg, length)
+    ea, rest = expr.split(",")
+    eb, ec = rest.split("->")
+    A = sht.Tensor(ctx, [sizes[x] for x in ea])
+    B = sht.Tensor(ctx, [sizes[x] for x in eb])
+    A.reserve_random(seed, 0, p)
+    B.reserve_random(seed, 1, p)
+    A.fill_random(seed, 0, lam)
+    B.fill_random(seed, 1, lam)
+    Cc = sht.Tensor(ctx, [sizes[x] for x in ec])
+    return expr, sizes, (ea, eb, ec), A, B, Cc
+
+
+def pick_sample(cfg, sizes, eabc, orc, target_flops):
+    """CPU sample: A restricted to a seeded set of A row groups (the
+    combined A free-index block coordinates), B whole.  The C blocks of those
+    row groups are exact copies of the full contraction's blocks (C blocks are
+    independent), so parity on them is exact."""
+    expr, L, fixed, p, lam, seed = CONFIGS[cfg]
+    ea, eb, ec = eabc
+    bsA = [sizes[x] for x in ea]
+    bsB = [sizes[x] for x in eb]
+    bsC = [sizes[x] for x in ec]
+    nA = np.array([len(b) for b in bsA], dtype=np.int64)
+    nB = np.array([len(b) for b in bsB], dtype=np.int64)
+    idsA = orc.gen_presence(seed, 0, nA, p)
+    idsB = orc.gen_presence(seed, 1, nB, p)
+    cA = [(idsA // s) % n for s, n in zip(np.cumprod(np.r_[1, nA[:-1]]), nA)]
+    cB = [(idsB // s) % n for s, n in zip(np.cumprod(np.r_[1, nB[:-1]]), nB)]
+    con = [x for x in ea if x in eb]
+    kA = np.zeros(len(idsA), np.int64)
+    kB = np.zeros(len(idsB), np.int64)
+    mul = 1
+    for x in con:
+        kA += cA[ea.index(x)] * mul
+        kB += cB[eb.index(x)] * mul
+        mul *= len(sizes[x])
+    volA = np.ones(len(idsA), np.float64)
+    for d, x in enumerate(ea):
+        volA *= np.asarray(sizes[x], np.float64)[cA[d]]
+    nfreeB = np.ones(len(idsB), np.float64)
+    for d, x in enumerate(eb):
+        if x not in ea:
+            nfreeB *= np.asarray(sizes[x], np.float64)[cB[d]]
+    S = np.bincount(kB, weights=nfreeB, minlength=mul)
+    flopsA = 2.0 * volA * S[kA]
+    row = np.zeros(len(idsA), np.int64)
+    mul = 1
+    for d, x in enumerate(ea):
+        if x not in eb:
+            row += cA[d] * mul
+            mul *= len(sizes[x])
+    rowflops = np.bincount(row, weights=flopsA, minlength=mul)
+    order = np.random.default_rng(seed).permutation(mul)
+    cum = np.cumsum(rowflops[order])
+    nrows = int(min(len(order), np.searchsorted(cum, target_flops) + 1))
+    chosen = order[:nrows]
+    As = orc.BlockSparse(bsA).reserve_ids(idsA[np.isin(row, chosen)])
+    orc.gen_values(seed, 0, As, lam)
+    B = orc.BlockSparse(bsB).reserve_ids(idsB)
+    orc.gen_values(seed, 1, B, lam)
+    Cs = orc.BlockSparse(bsC)
+    _, _, cids = orc.tasks(expr, As, B, Cs)
+    Cs.reserve_ids(cids)
+    return As, B, Cs, nrows, float(cum[nrows - 1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--length", type=int, default=None, help="override every mode length (smaller runs)")
+    ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time hipGraph replays instead of eager steps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import shtensor_amd as sht
+
+    t0 = time.time()
+    precs = {"fp64": sht.PREC_FP64, "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32,
+             "bf16x3": sht.PREC_BF16X3}
+    forms = [f for f in args.forms.split(",") if f]
+    arena = int(float(os.environ.get("SHT_BENCH_ARENA_GB", "200")) * (1 << 30))
+    ctx = sht.Context(arena)
+    expr, sizes, eabc, A, B, Cc = build_tensors(sht, ctx, args.config, args.length)
+    log("built tensors in %.1fs: A %d blocks (%d elems), B %d blocks" % (
+        time.time() - t0, A.get_nb_nzblocks_local(), A.nelem(), B.get_nb_nzblocks_local()))
+    stream = sht.Stream()
+    results = {}
+    ref_form = None
+    Ctens = {}
+    for f in forms:
+        Cf = sht.Tensor(ctx, Cc.block_sizes)
+        tp = time.time()
+        plan = sht.Plan(ctx, expr, A, B, Cf, precs[f])
+        st = plan.stats()
+        log("%s plan: %d tasks, %d C blocks, %.3e flop, %.2fs" % (f, st["ntasks"], st["ncblocks"], st["flops"],
+                                                                  time.time() - tp))
+        for _ in range(args.warmup):
+            plan.execute(1.0, 0.0, stream)
+        stream.synchronize()
+        times = []
+        if args.graph:
+            plan.graph_replay(1.0, 0.0, 1, stream)
+            stream.synchronize()
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            if args.graph:
+                plan.graph_replay(1.0, 0.0, 1, stream)
+            else:
+                plan.execute(1.0, 0.0, stream)
+            stream.synchronize()
+            times.append(time.perf_counter() - ts)
+        timed = plan.execute_timed(1.0, 0.0, stream)
+        times = np.array(times)
+        results[f] = {"stats": st, "times": times, "launches": timed, "plan": plan}
+        Ctens[f] = Cf
+        log("%s: median %.3f ms -> %.1f GFLOP/s; launches %s" % (
+            f, 1e3 * np.median(times), st["flops"] / np.median(times) / 1e9, timed))
+
+    # parity + CPU baseline on a sample of C blocks
+    parity = {}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        tgt = min(args.cpu_seconds * nthreads * 3e9, args.cpu_max_flops)
+        ts = time.time()
+        As, Bs, Cs, nrows, sflops = pick_sample(args.config, sizes, eabc, orc, tgt)
+        log("CPU sample: %d A row groups, %d A blocks, %d C blocks, %.3e flop (%.1fs to build)" % (
+            nrows, len(As.indices), len(Cs.indices), sflops, time.time() - ts))
+        tc = time.perf_counter()
+        ntask = orc.contract(expr, 1.0, As, Bs, 0.0, Cs, mode=0, nthreads=nthreads)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": round(sflops / cpu_s / 1e9, 3), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+               "sample": "%d of %d C blocks (every C block of %d seeded A row groups), %d tasks, %.3e flop, %.2f s" % (
+                   len(Cs.indices), results[forms[0]]["stats"]["ncblocks"], nrows, ntask, sflops, cpu_s)}
+        log("CPU: %s" % cpu)
+        ref = Cs.data
+        ends = np.r_[Cs.offsets[1:], len(ref)]
+        for f in forms:
+            Cf = Ctens[f]
+            info = Cf.get_sparse_info()
+            data = Cf.download()
+            pos = np.searchsorted(info["indices"], Cs.indices)
+            assert np.array_equal(info["indices"][pos], Cs.indices), "C pattern differs on the sample"
+            got = np.empty_like(ref)
+            mx = 0.0
+            for q, pp in enumerate(pos):
+                o0, o1 = Cs.offsets[q], ends[q]
+                g0 = info["offsets"][pp]
+                got[o0:o1] = data[g0:g0 + (o1 - o0)]
+                nb = np.linalg.norm(ref[o0:o1])
+                if nb > 0:
+                    mx = max(mx, float(np.linalg.norm(got[o0:o1] - ref[o0:o1]) / nb))
+            err = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+            parity[f] = {"normwise_rel": err, "max_blockwise_rel": mx, "bitwise": bool(np.array_equal(got, ref)),
+                         "cblocks": int(len(pos))}
+            log("parity %s: %s" % (f, parity[f]))
+
+    main_f = forms[0]
+    r = results[main_f]
+    med = float(np.median(r["times"]))
+    value = r["stats"]["flops"] / med / 1e9
+    # dominant kernel roofline (HIP events around each grouped launch)
+    dom = max(r["launches"], key=lambda x: x[1])
+    tot_ms = sum(ms for _, ms in r["launches"])
+    roof = {"bound": "mfma", "kernel": dom[0],
+            "achieved": round(r["stats"]["flops"] * (dom[1] / tot_ms) / (dom[1] * 1e-3) / 1e12, 3),
+            "peak": PEAK[main_f] / 1e12, "unit": "TFLOP/s", "traffic": None}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    out = {
+        "metric": "contraction GFLOP/s FP64 vs batched/FP32 at 1e-6 rel err; p50/p99 latency",
+        "value": round(value, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * med, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64, generated on the GPU)",
+        "config": {"workload": "%s %s L=%d" % (args.config, expr, args.length or CONFIGS[args.config][1]),
+                   "ntasks": r["stats"]["ntasks"], "ncblocks": r["stats"]["ncblocks"],
+                   "flops_per_step": r["stats"]["flops"], "graph": bool(args.graph)},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(r["times"], 50)), 4),
+                       "p99": round(1e3 * float(np.percentile(r["times"], 99)), 4)},
+        "forms": {f: {"gflops": round(results[f]["stats"]["flops"] / float(np.median(results[f]["times"])) / 1e9, 3),
+                      "ms_p50": round(1e3 * float(np.percentile(results[f]["times"], 50)), 4),
+                      "ms_p99": round(1e3 * float(np.percentile(results[f]["times"], 99)), 4),
+                      "launches": results[f]["launches"],
+                      "parity": parity.get(f)} for f in forms},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for f in forms:
+        results[f]["plan"].close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * sht.h — C ABI of shtensor_amd, an MI355X-native block-sparse tensor
+ * contraction with the contract of shtensor (Kernel<T>, Tensor<T,N>).
+ *
+ * All entry points return sht_status; on failure sht_last_error() holds the
+ * message (thread-local), using the reference's message text where the
+ * reference raises std::runtime_error.  No exception crosses this boundary.
+ * Device pointers are HIP device addresses; `stream` is a hipStream_t passed
+ * as void* (NULL = the null stream).  Layouts are column-major, first index
+ * fastest, as in the reference (src/Basics/Utils.h:80-94).
+ */
+#ifndef SHT_H
+#define SHT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SHT_OK = 0,
+  SHT_ERR_INVALID = 1,     /* bad argument                                      */
+  SHT_ERR_EXPR = 2,        /* contraction expression rejected (ContractInfo.h)   */
+  SHT_ERR_OOM = 3,         /* arena exhausted ("Out of memory", MemoryPool.h:148) */
+  SHT_ERR_HIP = 4,         /* HIP runtime or hipRTC failure                     */
+  SHT_ERR_STATE = 5,       /* object in the wrong state (e.g. not reserved)     */
+  SHT_ERR_UNSUPPORTED = 6, /* valid request this build does not implement       */
+  SHT_ERR_ADMISSION = 7    /* serving: request over the admission budget        */
+} sht_status;
+
+/* src/Basics/Definitions.h:45-49 */
+typedef enum { SHT_FLOAT32 = 0, SHT_FLOAT64 = 1 } sht_dtype;
+/* src/Kernel/KernelDefinitions.h:11-15 — accepted, single HIP backend */
+typedef enum { SHT_KERNEL_LAPACK = 0, SHT_KERNEL_XMM = 1 } sht_kernel_type;
+/* src/Basics/BlockSpan.h:87-91 */
+typedef enum { SHT_NORM_FROBENIUS = 0, SHT_NORM_MAXABS = 1 } sht_block_norm;
+
+/* Arithmetic of the block products of a contraction.                        */
+typedef enum {
+  SHT_PREC_FP64 = 0,       /* FP64 MFMA; <=1e-12 normwise vs the fma chain      */
+  SHT_PREC_FP64_EXACT = 1, /* FP64 fma chain in the reference's order: bitwise  */
+  SHT_PREC_FP32 = 2,       /* FP32 MFMA per block, FP64 across blocks           */
+  SHT_PREC_BF16X3 = 3      /* split-BF16 x3 MFMA per block, FP64 across blocks  */
+} sht_precision;
+
+typedef struct sht_ctx sht_ctx;
+typedef struct sht_kernel sht_kernel;
+typedef struct sht_tensor sht_tensor;
+typedef struct sht_plan sht_plan;
+
+const char* sht_last_error(void);
+const char* sht_version(void);
+
+/* ---- context + HBM arena (replaces Context + MemoryPool + Window:
+ *      src/Shmem/Context.cpp:9-48, MemoryPool.h:126-354, Window.h:11-71) ---- */
+sht_status sht_ctx_create(size_t arena_bytes, int device, sht_ctx** out);
+/* Host-backed context: same allocator over host memory, no device use.
+ * Lets hosts (and tests on machines without a GPU) use the arena
+ * accounting; tensors cannot be created in it.                              */
+sht_status sht_ctx_create_host(size_t arena_bytes, sht_ctx** out);
+sht_status sht_ctx_destroy(sht_ctx* ctx);
+sht_status sht_ctx_arena_info(const sht_ctx* ctx, size_t* total, size_t* free_bytes, size_t* largest_free);
+/* MemoryPool::allocate/reallocate/free (first fit, coalescing); 256-B aligned */
+sht_status sht_arena_alloc(sht_ctx* ctx, size_t bytes, void** dptr);
+sht_status sht_arena_realloc(sht_ctx* ctx, void* dptr, size_t bytes, void** out);
+sht_status sht_arena_free(sht_ctx* ctx, void* dptr);
+sht_status sht_arena_size(const sht_ctx* ctx, const void* dptr, size_t* bytes);
+
+/* ---- device helpers (plain HIP, for hosts without their own HIP code) ----- */
+sht_status sht_dev_alloc(size_t bytes, void** dptr);
+sht_status sht_dev_free(void* dptr);
+sht_status sht_copy_h2d(void* dst, const void* src, size_t bytes);
+sht_status sht_copy_d2h(void* dst, const void* src, size_t bytes);
+sht_status sht_device_synchronize(void);
+sht_status sht_stream_create(void** stream);
+sht_status sht_stream_destroy(void* stream);
+sht_status sht_stream_synchronize(void* stream);
+
+/* ---- block-product kernel (replaces Kernel<T>, src/Kernel/Kernel.h:37-93,
+ *      KernelImpl.cpp:9-73 ctor, 210-770 XMM JIT, 142-208 LAPACK) ----------
+ * sht_kernel_create: validates the expression exactly as ContractInfo.h:68-346
+ * and specialises a HIP kernel for the (expr, sizes, dtype, alpha, beta) key
+ * with hipRTC (cached per process and on disk).  alpha/beta are converted to
+ * dtype.  kernel_type is accepted for source compatibility only.          */
+sht_status sht_kernel_create(const char* expr, const int* sizes_a, int na, const int* sizes_b, int nb,
+                             const int* sizes_c, int nc, double alpha, double beta, sht_dtype dtype,
+                             sht_kernel_type kernel_type, sht_kernel** out);
+/* Kernel<T>::call (Kernel.h:88-91): nb_ops packed same-shape products,
+ * C_op = beta*C_op + sum_k (alpha*A_op) B_op; a/b/c are device pointers.
+ * nb_ops <= 0 is a no-op.  Asynchronous on `stream`.                        */
+sht_status sht_kernel_call(sht_kernel* k, const void* a, const void* b, void* c, int64_t nb_ops, void* stream);
+/* KernelImpl::get_info (KernelImpl.cpp:773-794) text into buf.             */
+sht_status sht_kernel_info(const sht_kernel* k, char* buf, size_t len);
+sht_status sht_kernel_destroy(sht_kernel* k);
+/* Compile a kernel's code object into the disk cache without a GPU.        */
+sht_status sht_kernel_prewarm(const char* expr, const int* sizes_a, int na, const int* sizes_b, int nb,
+                              const int* sizes_c, int nc, double alpha, double beta, sht_dtype dtype);
+/* compute_contract_info (src/Basics/ContractInfo.h:68-346), host only.
+ * maps_out: 6 records of 17 ints (len, v0..v15): A.row, A.col, B.row, B.col,
+ * C.row, C.col.  Fails with SHT_ERR_EXPR and the reference's message.      */
+sht_status sht_contract_info(const char* expr, const int* sizes_a, int na, const int* sizes_b, int nb,
+                             const int* sizes_c, int nc, int* maps_out);
+
+/* ---- block-sparse tensor (replaces Tensor<T,N>/BaseTensor,
+ *      src/Tensor/Tensor.h:27-294, BaseTensor.h:27-53, BaseTensor.cpp) ----- */
+/* block_sizes: N pointers to nblk[d] block lengths.                        */
+sht_status sht_tensor_create(sht_ctx* ctx, sht_dtype dtype, int N, const int* const* block_sizes,
+                             const int* nblk, sht_tensor** out);
+sht_status sht_tensor_destroy(sht_tensor* t);
+/* BaseTensor::reserve (BaseTensor.cpp:221-355): N pointers to nb block
+ * coordinates; ids sorted ascending, packed zeroed data in the arena.
+ * Errors: "reserve: Indices do not have the same size in each dimension"
+ * cannot arise through this signature; duplicates and out-of-range
+ * coordinates are rejected.                                                 */
+sht_status sht_tensor_reserve(sht_tensor* t, const int* const* idx_per_mode, int64_t nb);
+/* reserve by rolled block ids (any order)                                  */
+sht_status sht_tensor_reserve_ids(sht_tensor* t, const int64_t* ids, int64_t nb);
+sht_status sht_tensor_reserve_all(sht_tensor* t);
+/* host views valid until the next reserve/filter/compress of t             */
+sht_status sht_tensor_sparse_info(const sht_tensor* t, const int64_t** indices, const int64_t** offsets,
+                                  const int64_t** rows, int64_t* nnz);
+sht_status sht_tensor_data(const sht_tensor* t, void** dptr, int64_t* nelem);
+sht_status sht_tensor_nb_nzblocks(const sht_tensor* t, int64_t* local, int64_t* global);
+sht_status sht_tensor_occupation(const sht_tensor* t, double* occ);
+/* copy the whole packed data from/to host memory                           */
+sht_status sht_tensor_upload(sht_tensor* t, const void* host, int64_t nelem);
+sht_status sht_tensor_download(const sht_tensor* t, void* host, int64_t nelem);
+/* Per-block norms into a device array of nnz doubles (FP64 accumulation).  */
+sht_status sht_tensor_block_norms(const sht_tensor* t, sht_block_norm norm, double* d_out, void* stream);
+/* Tensor::filter (Tensor.h:336-363): block ids with norm < eps are negated
+ * (-id-1); compress!=0 then compacts (BaseTensor::compress, fixed).        */
+sht_status sht_tensor_filter(sht_tensor* t, double eps, sht_block_norm norm, int compress);
+sht_status sht_tensor_compress(sht_tensor* t);
+
+/* ---- synthetic workloads (counter-based SplitMix64; identical on CPU) ---- */
+sht_status sht_gen_block_sizes(uint64_t seed, int axis, int length, const int* choices, int nchoices, int* out,
+                               int cap, int* count);
+/* reserve the Bernoulli(p) pattern of tensor `tid`                          */
+sht_status sht_gen_reserve(sht_tensor* t, uint64_t seed, int tid, double p);
+/* fill values u(-1,1) * exp(-lambda*|c0-c1|) on the device                  */
+sht_status sht_gen_fill(sht_tensor* t, uint64_t seed, int tid, double lambda, void* stream);
+
+/* ---- contraction (the driver the reference lacks: src/Tensor/Tensor.cpp is
+ *      empty; semantics follow the Kernel's per-element fma chain) ----------
+ * C = beta*C + alpha * contract(expr, A, B).  If C holds no blocks its
+ * pattern is built from the product (sorted unique product blocks) and
+ * reserved; otherwise C's pattern is kept and products outside it dropped.
+ * The task list and pattern are built on the GPU.                           */
+sht_status sht_contract(sht_ctx* ctx, const char* expr, double alpha, const sht_tensor* A, const sht_tensor* B,
+                        double beta, sht_tensor* C, sht_precision prec, void* stream);
+
+/* Plans: build once (task list, C pattern, schedule), execute many times.  */
+sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, const sht_tensor* B,
+                           sht_tensor* C, sht_precision prec, sht_plan** out);
+sht_status sht_plan_execute(sht_plan* p, double alpha, double beta, void* stream);
+/* Execute once with a HIP event pair around every grouped launch (on
+ * `stream`), wait, and report each launch's kernel class (kclass_out) and
+ * duration in ms.  *nlaunch receives the launch count (<= cap written).     */
+sht_status sht_plan_execute_timed(sht_plan* p, double alpha, double beta, void* stream, int* kclass_out,
+                                  float* ms_out, int cap, int* nlaunch);
+/* Name of a kernel class reported by sht_plan_execute_timed.               */
+const char* sht_kclass_name(int kclass);
+/* capture one execution into a hipGraph and replay it `replays` times       */
+sht_status sht_plan_graph_replay(sht_plan* p, double alpha, double beta, int replays, void* stream);
+/* task list (host copies): positions into A, B, C and the contracted-block
+ * key, sorted by (C id, key).  Arrays valid until the plan is destroyed.     */
+sht_status sht_plan_tasks(sht_plan* p, const int64_t** ta, const int64_t** tb, const int64_t** tc,
+                          const int64_t** kkey, int64_t* ntasks);
+/* useful flops (sum 2*m*n*k) and algorithmic HBM bytes of one execution     */
+sht_status sht_plan_stats(const sht_plan* p, double* flops, double* bytes, int64_t* ntasks, int64_t* ncblocks);
+sht_status sht_plan_destroy(sht_plan* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,441 @@
+// Execution kernels of a contraction plan (C-stationary grouped launches).
+//
+// Every kernel owns one tile of one C block and walks that block's tasks in
+// ascending contracted-block key, so C needs no atomics and every element
+// sees its contributions in one fixed order.  Operands are the canonical
+// column-major matrices of each block: A (m x k), B (k x n), C (m x n).
+//
+//  * k_exact<NC>  — the reference JIT's per-element fma chain
+//    (src/Kernel/KernelImpl.cpp:594-710) continued across tasks: bitwise equal
+//    to the CPU chain.  One wave, 2 rows per lane (128 rows), NC columns held
+//    in registers; B chunks staged in LDS and read as wave-wide broadcasts.
+//  * k_mfma64<CT> — v_mfma_f64_16x16x4f64, 64 rows x 16*CT columns per wave,
+//    accumulators persist across tasks (FP64 throughout).
+//  * k_mfma32<CT> — v_mfma_f32_16x16x4f32 per task, task partial added into
+//    an FP64 accumulator (FP64 across blocks).
+//  * k_bf16x3<CT> — split-BF16 (hi + lo) products hi*hi + hi*lo + lo*hi with
+//    v_mfma_f32_16x16x32_bf16, FP32 per task, FP64 across tasks.
+
+#include "plan.hpp"
+
+namespace sht {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// bit-exact fma chain
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ __launch_bounds__(64) void k_exact(ExecArgs p)
+{
+  constexpr int R = 2;
+  constexpr int KC = 32;
+  __shared__ double Bs[KC * NC];
+  const TileRec tile = p.tiles[blockIdx.x];
+  const CBlkRec cb = p.cblk[tile.cb];
+  const int M = cb.m, N = cb.n, n0 = tile.n0;
+  const int nc = min(NC, N - n0);
+  const int lane = threadIdx.x;
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  double* Cb = static_cast<double*>(p.C) + cb.c_off;
+  int row[R];
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    row[r] = tile.m0 + lane + r * kWave;
+    ok[r] = row[r] < M;
+  }
+  double acc[R][NC];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      acc[r][j] = (!p.beta_zero && ok[r] && j < nc) ? Cb[row[r] + static_cast<int64_t>(n0 + j) * M] * p.beta : 0.0;
+  if (!p.alpha_zero) {
+    for (int t = cb.t0; t < cb.t1; ++t) {
+      const TaskRec tr = p.tasks[t];
+      const double* Ab = A + tr.a_off;
+      const double* Bb = B + tr.b_off;
+      const int K = tr.k;
+      for (int k0 = 0; k0 < K; k0 += KC) {
+        const int kc = min(KC, K - k0);
+        __syncthreads();
+        for (int e = lane; e < KC * NC; e += kWave) {
+          const int kk = e % KC, j = e / KC;
+          Bs[kk * NC + j] = (kk < kc && j < nc) ? Bb[(k0 + kk) + static_cast<int64_t>(n0 + j) * K] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int kk = 0; kk < kc; ++kk) {
+          double a[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            a[r] = ok[r] ? Ab[row[r] + static_cast<int64_t>(k0 + kk) * M] : 0.0;
+            if (!p.alpha_one) a[r] = a[r] * p.alpha;
+          }
+#pragma unroll
+          for (int j = 0; j < NC; ++j) {
+            const double b = Bs[kk * NC + j];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r][j] = __fma_rn(a[r], b, acc[r][j]);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      if (ok[r] && j < nc) Cb[row[r] + static_cast<int64_t>(n0 + j) * M] = acc[r][j];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA.  Operand layout of v_mfma_f64_16x16x4f64 (wave64):
+//   A frag: lane l holds A[l%16][l/16]; B frag: B[l/16][l%16];
+//   D: lane l, item v holds D[4*(l/16)+v][l%16].
+// ---------------------------------------------------------------------------
+template <int CT>
+__global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
+{
+  constexpr int RT = 4;
+  const TileRec tile = p.tiles[blockIdx.x];
+  const CBlkRec cb = p.cblk[tile.cb];
+  const int M = cb.m, N = cb.n;
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  double* Cb = static_cast<double*>(p.C) + cb.c_off;
+  v4d acc[RT][CT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int j = tile.n0 + 16 * ct + li;
+        acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
+      }
+  if (!p.alpha_zero) {
+    for (int t = cb.t0; t < cb.t1; ++t) {
+      const TaskRec tr = p.tasks[t];
+      const double* Ab = A + tr.a_off;
+      const double* Bb = B + tr.b_off;
+      const int K = tr.k;
+      for (int k = 0; k < K; k += 4) {
+        const int kk = k + lk;
+        const bool kin = kk < K;
+        double a[RT], b[CT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          const int i = tile.m0 + 16 * rt + li;
+          a[rt] = (kin && i < M) ? Ab[i + static_cast<int64_t>(kk) * M] : 0.0;
+          if (!p.alpha_one) a[rt] *= p.alpha;
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int j = tile.n0 + 16 * ct + li;
+          b[ct] = (kin && j < N) ? Bb[kk + static_cast<int64_t>(j) * K] : 0.0;
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int j = tile.n0 + 16 * ct + li;
+        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA per task, FP64 across tasks.  v_mfma_f32_16x16x4f32 has the same
+// operand/result layout as the f64 variant above.
+// ---------------------------------------------------------------------------
+template <int CT>
+__global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
+{
+  constexpr int RT = 4;
+  const TileRec tile = p.tiles[blockIdx.x];
+  const CBlkRec cb = p.cblk[tile.cb];
+  const int M = cb.m, N = cb.n;
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  double* Cb = static_cast<double*>(p.C) + cb.c_off;
+  v4d acc[RT][CT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int j = tile.n0 + 16 * ct + li;
+        acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
+      }
+  if (!p.alpha_zero) {
+    for (int t = cb.t0; t < cb.t1; ++t) {
+      const TaskRec tr = p.tasks[t];
+      const double* Ab = A + tr.a_off;
+      const double* Bb = B + tr.b_off;
+      const int K = tr.k;
+      v4f part[RT][CT];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int k = 0; k < K; k += 4) {
+        const int kk = k + lk;
+        const bool kin = kk < K;
+        float a[RT], b[CT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          const int i = tile.m0 + 16 * rt + li;
+          a[rt] = (kin && i < M) ? static_cast<float>(Ab[i + static_cast<int64_t>(kk) * M]) : 0.f;
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int j = tile.n0 + 16 * ct + li;
+          b[ct] = (kin && j < N) ? static_cast<float>(Bb[kk + static_cast<int64_t>(j) * K]) : 0.f;
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], part[rt][ct], 0, 0, 0);
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int j = tile.n0 + 16 * ct + li;
+        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// split-BF16 x3.  v_mfma_f32_16x16x32_bf16 (wave64): A frag lane l holds
+// A[l%16][8*(l/16) .. 8*(l/16)+7]; B frag holds B[8*(l/16)..+7][l%16];
+// D as above.  x = hi + lo with hi = bf16(x), lo = bf16(x - hi); the product
+// keeps hi*hi + hi*lo + lo*hi (lo*lo dropped), accumulated in FP32 per task.
+// ---------------------------------------------------------------------------
+__device__ inline unsigned short bf16_rne(float x)
+{
+  unsigned int u = __float_as_uint(x);
+  const unsigned int r = ((u >> 16) & 1u) + 0x7FFFu;
+  return static_cast<unsigned short>((u + r) >> 16);
+}
+__device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
+
+template <int CT>
+__global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
+{
+  constexpr int RT = 4;
+  const TileRec tile = p.tiles[blockIdx.x];
+  const CBlkRec cb = p.cblk[tile.cb];
+  const int M = cb.m, N = cb.n;
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  double* Cb = static_cast<double*>(p.C) + cb.c_off;
+  v4d acc[RT][CT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int j = tile.n0 + 16 * ct + li;
+        acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
+      }
+  if (!p.alpha_zero) {
+    for (int t = cb.t0; t < cb.t1; ++t) {
+      const TaskRec tr = p.tasks[t];
+      const double* Ab = A + tr.a_off;
+      const double* Bb = B + tr.b_off;
+      const int K = tr.k;
+      v4f part[RT][CT];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int k = 0; k < K; k += 32) {
+        v8s ahi[RT], alo[RT], bhi[CT], blo[CT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          const int i = tile.m0 + 16 * rt + li;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int kk = k + 8 * lk + e;
+            const float x = (kk < K && i < M) ? static_cast<float>(Ab[i + static_cast<int64_t>(kk) * M]) : 0.f;
+            const unsigned short h = bf16_rne(x);
+            ahi[rt][e] = static_cast<short>(h);
+            alo[rt][e] = static_cast<short>(bf16_rne(x - bf16_to_f(h)));
+          }
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int j = tile.n0 + 16 * ct + li;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int kk = k + 8 * lk + e;
+            const float x = (kk < K && j < N) ? static_cast<float>(Bb[kk + static_cast<int64_t>(j) * K]) : 0.f;
+            const unsigned short h = bf16_rne(x);
+            bhi[ct][e] = static_cast<short>(h);
+            blo[ct][e] = static_cast<short>(bf16_rne(x - bf16_to_f(h)));
+          }
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct) {
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rt], bhi[ct], part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt], blo[ct], part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rt], bhi[ct], part[rt][ct], 0, 0, 0);
+          }
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int j = tile.n0 + 16 * ct + li;
+        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// block permutation (Utils::reshape per block)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_permute(PermArgs p)
+{
+  const TensorDesc& d = p.d;
+  for (int64_t q = blockIdx.x; q < d.nnz; q += gridDim.x) {
+    int64_t id = d.indices[q];
+    int dims[kMaxModes];
+    for (int m = d.N - 1; m >= 0; --m) {
+      const int c = static_cast<int>(id / d.bstride[m]);
+      id -= static_cast<int64_t>(c) * d.bstride[m];
+      dims[m] = d.bsizes[d.boff[m] + c];
+    }
+    int64_t sstride[kMaxModes];
+    int64_t vol = 1;
+    for (int m = 0; m < d.N; ++m) {
+      sstride[m] = vol;
+      vol *= dims[m];
+    }
+    const T* in = static_cast<const T*>(p.in) + d.offsets[q];
+    T* out = static_cast<T*>(p.out) + d.offsets[q];
+    for (int64_t x = threadIdx.x; x < vol; x += blockDim.x) {
+      // x indexes the canonical layout (dims reordered by `order`)
+      int64_t rem = x, src = 0;
+      for (int i = 0; i < d.N; ++i) {
+        const int ext = dims[p.order[i]];
+        const int64_t ci = rem % ext;
+        rem /= ext;
+        src += ci * sstride[p.order[i]];
+      }
+      if (p.inverse) out[src] = in[x];
+      else out[x] = in[src];
+    }
+  }
+}
+
+}  // namespace
+
+int kclass_rows(KClass k)
+{
+  switch (k) {
+    case KClass::Exact8:
+    case KClass::Exact16:
+    case KClass::Exact32: return 128;
+    default: return 64;
+  }
+}
+
+int kclass_cols(KClass k)
+{
+  switch (k) {
+    case KClass::Exact8: return 8;
+    case KClass::Exact16: return 16;
+    case KClass::Exact32: return 32;
+    case KClass::Mfma16:
+    case KClass::F32x16:
+    case KClass::Bf16x16: return 16;
+    default: return 32;
+  }
+}
+
+void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
+{
+  if (a.ntiles <= 0) return;
+  const dim3 g(static_cast<unsigned>(a.ntiles)), b(kWave);
+  switch (k) {
+    case KClass::Exact8: hipLaunchKernelGGL(k_exact<8>, g, b, 0, s, a); break;
+    case KClass::Exact16: hipLaunchKernelGGL(k_exact<16>, g, b, 0, s, a); break;
+    case KClass::Exact32: hipLaunchKernelGGL(k_exact<32>, g, b, 0, s, a); break;
+    case KClass::Mfma16: hipLaunchKernelGGL(k_mfma64<1>, g, b, 0, s, a); break;
+    case KClass::Mfma32: hipLaunchKernelGGL(k_mfma64<2>, g, b, 0, s, a); break;
+    case KClass::F32x16: hipLaunchKernelGGL(k_mfma32<1>, g, b, 0, s, a); break;
+    case KClass::F32x32: hipLaunchKernelGGL(k_mfma32<2>, g, b, 0, s, a); break;
+    case KClass::Bf16x16: hipLaunchKernelGGL(k_bf16x3<1>, g, b, 0, s, a); break;
+    case KClass::Bf16x32: hipLaunchKernelGGL(k_bf16x3<2>, g, b, 0, s, a); break;
+    default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
+  }
+  SHT_HIP(hipGetLastError());
+}
+
+void launch_permute(const PermArgs& p, bool f64, hipStream_t s)
+{
+  if (p.d.nnz <= 0) return;
+  const int g = static_cast<int>(p.d.nnz < 65536 ? p.d.nnz : 65536);
+  if (f64) hipLaunchKernelGGL(k_permute<double>, dim3(g), dim3(256), 0, s, p);
+  else hipLaunchKernelGGL(k_permute<float>, dim3(g), dim3(256), 0, s, p);
+  SHT_HIP(hipGetLastError());
+}
+
+}  // namespace sht

@@ -1,0 +1,811 @@
+// Contraction driver: GPU task list + C pattern, plan, execution, graphs.
+//
+// The reference has no contraction driver (src/Tensor/Tensor.cpp is empty);
+// its building blocks define the rule implemented here:
+//   * block-level expression analysis = ContractInfo on the block counts
+//     (src/Basics/ContractInfo.h:68-346);
+//   * task = (A block, B block) with equal contracted block coordinates,
+//     keyed by the contracted coordinates rolled in A's axis order;
+//   * tasks sorted by (C block id, key); C pattern = sorted unique C ids
+//     (the storage order of BaseTensor::reserve, BaseTensor.cpp:270-312);
+//   * per C element: beta*C (not read when beta == +0), then the Kernel's
+//     fma chain (KernelImpl.cpp:594-710) over the tasks in key order.
+// Built on the device: key extraction, radix sort of B by key, binary-search
+// ranges, scan + expansion of the pairs, radix sort of (C id, key), head-flag
+// scan for the unique C ids, and per-C-block task ranges.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <cmath>
+#include <numeric>
+
+#include "plan.hpp"
+
+namespace sht {
+
+namespace {
+
+struct KeySpec {
+  int nk;
+  int kax[kMaxModes];
+  long long kmul[kMaxModes];
+  int nf;
+  int fax[kMaxModes];
+  long long fmul[kMaxModes];
+};
+
+__device__ inline void coords_of(const TensorDesc& d, int64_t id, int* c)
+{
+  for (int m = d.N - 1; m >= 0; --m) {
+    c[m] = static_cast<int>(id / d.bstride[m]);
+    id -= static_cast<int64_t>(c[m]) * d.bstride[m];
+  }
+}
+
+__global__ void k_keys(TensorDesc d, KeySpec ks, int64_t* kkey, int64_t* cpart)
+{
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= d.nnz) return;
+  int c[kMaxModes];
+  coords_of(d, d.indices[p], c);
+  int64_t k = 0, f = 0;
+  for (int r = 0; r < ks.nk; ++r) k += static_cast<int64_t>(c[ks.kax[r]]) * ks.kmul[r];
+  for (int r = 0; r < ks.nf; ++r) f += static_cast<int64_t>(c[ks.fax[r]]) * ks.fmul[r];
+  kkey[p] = k;
+  cpart[p] = f;
+}
+
+__global__ void k_iota(int64_t* x, int64_t n)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = i;
+}
+
+__device__ inline int64_t lower_bound_dev(const int64_t* a, int64_t n, int64_t v)
+{
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_ranges(const int64_t* kA, int64_t nA, const int64_t* kBs, int64_t nB, int64_t* lo, int64_t* cnt)
+{
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= nA) return;
+  const int64_t l = lower_bound_dev(kBs, nB, kA[p]);
+  const int64_t h = lower_bound_dev(kBs, nB, kA[p] + 1);
+  lo[p] = l;
+  cnt[p] = h - l;
+}
+
+__global__ void k_expand(const int64_t* lo, const int64_t* cnt, const int64_t* toff, int64_t nA, const int64_t* permB,
+                         const int64_t* cA, const int64_t* cB, const int64_t* kA, int64_t nK, int64_t* ta,
+                         int64_t* tb, uint64_t* key)
+{
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= nA) return;
+  const int64_t base = toff[p], n = cnt[p], l = lo[p];
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t b = permB[l + q];
+    ta[base + q] = p;
+    tb[base + q] = b;
+    key[base + q] = static_cast<uint64_t>((cA[p] + cB[b]) * nK + kA[p]);
+  }
+}
+
+__global__ void k_gather(const int64_t* order, int64_t n, const int64_t* ta, const int64_t* tb, const uint64_t* key,
+                         int64_t nK, int64_t* ta_s, int64_t* tb_s, int64_t* cid_s, int64_t* kk_s)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t o = order[t];
+  ta_s[t] = ta[o];
+  tb_s[t] = tb[o];
+  const uint64_t k = key[t];
+  cid_s[t] = static_cast<int64_t>(k / static_cast<uint64_t>(nK));
+  kk_s[t] = static_cast<int64_t>(k % static_cast<uint64_t>(nK));
+}
+
+__global__ void k_headflags(const int64_t* cid, int64_t n, int64_t* flag)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < n) flag[t] = (t == 0 || cid[t] != cid[t - 1]) ? 1 : 0;
+}
+
+__global__ void k_unique(const int64_t* cid, const int64_t* flag, int64_t* incl, int64_t n, int64_t* uids)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t pos = incl[t] - 1;
+  if (flag[t]) uids[pos] = cid[t];
+  incl[t] = pos;  // becomes the task's C position
+}
+
+__global__ void k_lookup(const int64_t* cid, int64_t n, const int64_t* cind, int64_t nc, int64_t* tc, char* keep)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t pos = lower_bound_dev(cind, nc, cid[t]);
+  const bool found = pos < nc && cind[pos] == cid[t];
+  tc[t] = found ? pos : -1;
+  keep[t] = found ? 1 : 0;
+}
+
+__global__ void k_cstart(const int64_t* tc, int64_t nt, int64_t ncb, int64_t* cstart)
+{
+  const int64_t pc = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pc <= ncb) cstart[pc] = lower_bound_dev(tc, nt, pc);
+}
+
+__device__ inline int64_t prod_axes(const TensorDesc& d, const int* c, const int* ax, int n)
+{
+  int64_t v = 1;
+  for (int r = 0; r < n; ++r) v *= d.bsizes[d.boff[ax[r]] + c[ax[r]]];
+  return v;
+}
+
+struct AxisList {
+  int n;
+  int ax[kMaxModes];
+};
+
+__global__ void k_task_recs(const int64_t* ta, const int64_t* tb, int64_t nt, TensorDesc A, TensorDesc B,
+                            AxisList acol, TaskRec* out)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int64_t a = ta[t], b = tb[t];
+  int c[kMaxModes];
+  coords_of(A, A.indices[a], c);
+  TaskRec r;
+  r.a_off = A.offsets[a];
+  r.b_off = B.offsets[b];
+  r.k = static_cast<int32_t>(prod_axes(A, c, acol.ax, acol.n));
+  r.pad = 0;
+  out[t] = r;
+}
+
+__global__ void k_cblk_recs(TensorDesc C, AxisList crow, AxisList ccol, const int64_t* cstart, const TaskRec* tasks,
+                            CBlkRec* out, int64_t* work)
+{
+  const int64_t pc = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pc >= C.nnz) return;
+  int c[kMaxModes];
+  coords_of(C, C.indices[pc], c);
+  CBlkRec r;
+  r.c_off = C.offsets[pc];
+  r.m = static_cast<int32_t>(prod_axes(C, c, crow.ax, crow.n));
+  r.n = static_cast<int32_t>(prod_axes(C, c, ccol.ax, ccol.n));
+  r.t0 = static_cast<int32_t>(cstart[pc]);
+  r.t1 = static_cast<int32_t>(cstart[pc + 1]);
+  int64_t w = 0;
+  for (int64_t t = r.t0; t < r.t1; ++t) w += tasks[t].k;
+  out[pc] = r;
+  work[pc] = w;
+}
+
+inline unsigned blocks_for(int64_t n, int tpb = 256) { return static_cast<unsigned>((std::max<int64_t>(n, 1) + tpb - 1) / tpb); }
+
+int bits_for(uint64_t maxval)
+{
+  int b = 1;
+  while (b < 64 && (maxval >> b)) ++b;
+  return b;
+}
+
+// Scoped arena allocation for temporaries.
+class Scratch {
+ public:
+  explicit Scratch(Arena& a) : a_(a) {}
+  ~Scratch()
+  {
+    for (void* p : ptrs_) a_.release(p);
+  }
+  template <class T>
+  T* get(int64_t n)
+  {
+    void* p = a_.allocate(static_cast<size_t>(std::max<int64_t>(n, 1)) * sizeof(T));
+    ptrs_.push_back(p);
+    return static_cast<T*>(p);
+  }
+  void* bytes(size_t n)
+  {
+    void* p = a_.allocate(std::max<size_t>(n, 1));
+    ptrs_.push_back(p);
+    return p;
+  }
+
+ private:
+  Arena& a_;
+  std::vector<void*> ptrs_;
+};
+
+bool canonical(const std::vector<int>& lead, const std::vector<int>& trail)
+{
+  for (size_t r = 0; r < lead.size(); ++r)
+    if (lead[r] != static_cast<int>(r)) return false;
+  for (size_t r = 0; r < trail.size(); ++r)
+    if (trail[r] != static_cast<int>(lead.size() + r)) return false;
+  return true;
+}
+
+AxisList axis_list(const std::vector<int>& v)
+{
+  AxisList a{};
+  a.n = static_cast<int>(v.size());
+  for (size_t i = 0; i < v.size(); ++i) a.ax[i] = v[i];
+  return a;
+}
+
+}  // namespace
+
+}  // namespace sht
+
+struct sht_plan {
+  sht_ctx* ctx = nullptr;
+  std::string expr;
+  const sht_tensor* A = nullptr;
+  const sht_tensor* B = nullptr;
+  sht_tensor* C = nullptr;
+  sht_precision prec = SHT_PREC_FP64;
+  sht::ContractMaps maps;
+  int64_t ntasks = 0, ncb = 0;
+  // device (arena)
+  int64_t* d_ta = nullptr;
+  int64_t* d_tb = nullptr;
+  int64_t* d_tc = nullptr;
+  int64_t* d_kk = nullptr;
+  sht::TaskRec* d_tasks = nullptr;
+  sht::CBlkRec* d_cblk = nullptr;
+  sht::TileRec* d_tiles = nullptr;
+  struct Launch {
+    sht::KClass k;
+    int64_t first, count;
+  };
+  std::vector<Launch> launches;
+  // staging for non-canonical layouts
+  bool permA = false, permB = false, permC = false;
+  void* d_Ap = nullptr;
+  void* d_Bp = nullptr;
+  void* d_Cp = nullptr;
+  int orderA[sht::kMaxModes], orderB[sht::kMaxModes], orderC[sht::kMaxModes];
+  // host copies
+  std::vector<int64_t> h_ta, h_tb, h_tc, h_kk;
+  bool host_tasks = false;
+  double flops = 0.0;
+  double bytes_in = 0.0;  // A + B + C write
+  double bytes_c = 0.0;   // C read (beta != 0)
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  double graph_alpha = 0.0, graph_beta = 0.0;
+
+  void release()
+  {
+    sht::Arena& a = ctx->arena;
+    for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
+                    static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
+                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp})
+      if (p) a.release(p);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_def) (void)hipGraphDestroy(graph_def);
+  }
+};
+
+namespace sht {
+
+namespace {
+
+void check_tensors(const sht_tensor* A, const sht_tensor* B, const sht_tensor* C, sht_ctx* ctx)
+{
+  SHT_REQUIRE(A && B && C, SHT_ERR_INVALID, "null tensor");
+  SHT_REQUIRE(A->reserved && B->reserved, SHT_ERR_STATE, "A and B must be reserved");
+  SHT_REQUIRE(A->ctx == ctx && B->ctx == ctx && C->ctx == ctx, SHT_ERR_INVALID, "tensors of another context");
+  SHT_REQUIRE(A->dtype == SHT_FLOAT64 && B->dtype == SHT_FLOAT64 && C->dtype == SHT_FLOAT64, SHT_ERR_UNSUPPORTED,
+              "contraction supports FLOAT64 tensors (precision forms select the block arithmetic)");
+  SHT_REQUIRE(A != C && B != C, SHT_ERR_INVALID, "C must not alias A or B");
+}
+
+// Build the task list and C pattern on the device.
+void build_tasks(sht_plan* P)
+{
+  const sht_tensor* A = P->A;
+  const sht_tensor* B = P->B;
+  sht_tensor* C = P->C;
+  const ContractMaps& m = P->maps;
+  Arena& arena = P->ctx->arena;
+  Scratch S(arena);
+  hipStream_t s = nullptr;
+
+  // keys: contracted coords rolled in A's con order; C id split into the part
+  // carried by A's free coords and the part carried by B's free coords
+  KeySpec ka{}, kb{};
+  int64_t nK = 1;
+  ka.nk = kb.nk = static_cast<int>(m.a_col.size());
+  for (size_t r = 0; r < m.a_col.size(); ++r) {
+    ka.kax[r] = m.a_col[r];
+    kb.kax[r] = m.b_row[r];
+    ka.kmul[r] = kb.kmul[r] = nK;
+    nK *= A->nblk[m.a_col[r]];
+  }
+  ka.nf = static_cast<int>(m.a_row.size());
+  for (size_t r = 0; r < m.a_row.size(); ++r) {
+    ka.fax[r] = m.a_row[r];
+    ka.fmul[r] = C->bstride[m.c_row[r]];
+  }
+  kb.nf = static_cast<int>(m.b_col.size());
+  for (size_t r = 0; r < m.b_col.size(); ++r) {
+    kb.fax[r] = m.b_col[r];
+    kb.fmul[r] = C->bstride[m.c_col[r]];
+  }
+  const double comp = static_cast<double>(C->nblocks_total) * static_cast<double>(nK);
+  SHT_REQUIRE(comp < 9.0e18, SHT_ERR_UNSUPPORTED, "block index space too large for 64-bit task keys");
+
+  const TensorDesc dA = A->desc(), dB = B->desc();
+  const int64_t nA = dA.nnz, nB = dB.nnz;
+  P->ntasks = 0;
+  if (nA == 0 || nB == 0) {
+    if (C->indices.empty() && !C->reserved) tensor_reserve_sorted(C, {});
+    else if (!C->reserved) tensor_reserve_sorted(C, {});
+    P->ncb = static_cast<int64_t>(C->indices.size());
+    return;
+  }
+  int64_t* kA = S.get<int64_t>(nA);
+  int64_t* cA = S.get<int64_t>(nA);
+  int64_t* kB = S.get<int64_t>(nB);
+  int64_t* cB = S.get<int64_t>(nB);
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(nA)), dim3(256), 0, s, dA, ka, kA, cA);
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(nB)), dim3(256), 0, s, dB, kb, kB, cB);
+  SHT_HIP(hipGetLastError());
+
+  // sort B by key
+  int64_t* kBs = S.get<int64_t>(nB);
+  int64_t* iotaB = S.get<int64_t>(nB);
+  int64_t* permB = S.get<int64_t>(nB);
+  hipLaunchKernelGGL(k_iota, dim3(blocks_for(nB)), dim3(256), 0, s, iotaB, nB);
+  const int kbits = bits_for(static_cast<uint64_t>(nK));
+  size_t tmp = 0;
+  SHT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kB, kBs, iotaB, permB, static_cast<int>(nB), 0, kbits, s));
+  SHT_HIP(hipcub::DeviceRadixSort::SortPairs(S.bytes(tmp), tmp, kB, kBs, iotaB, permB, static_cast<int>(nB), 0,
+                                             kbits, s));
+
+  // matching B range of every A block, then task offsets
+  int64_t* lo = S.get<int64_t>(nA);
+  int64_t* cnt = S.get<int64_t>(nA);
+  int64_t* toff = S.get<int64_t>(nA);
+  hipLaunchKernelGGL(k_ranges, dim3(blocks_for(nA)), dim3(256), 0, s, kA, nA, kBs, nB, lo, cnt);
+  tmp = 0;
+  SHT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, toff, static_cast<int>(nA), s));
+  SHT_HIP(hipcub::DeviceScan::ExclusiveSum(S.bytes(tmp), tmp, cnt, toff, static_cast<int>(nA), s));
+  int64_t last_off = 0, last_cnt = 0;
+  SHT_HIP(hipMemcpy(&last_off, toff + nA - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+  SHT_HIP(hipMemcpy(&last_cnt, cnt + nA - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+  int64_t nt = last_off + last_cnt;
+  SHT_REQUIRE(nt < (int64_t(1) << 31), SHT_ERR_UNSUPPORTED, "more than 2^31 block products in one contraction");
+
+  const bool keep = C->reserved && !C->indices.empty();
+  if (nt == 0) {
+    if (!keep) tensor_reserve_sorted(C, {});
+    P->ncb = static_cast<int64_t>(C->indices.size());
+    P->d_cblk = nullptr;
+    return;
+  }
+  int64_t* ta = S.get<int64_t>(nt);
+  int64_t* tb = S.get<int64_t>(nt);
+  uint64_t* key = S.get<uint64_t>(nt);
+  hipLaunchKernelGGL(k_expand, dim3(blocks_for(nA)), dim3(256), 0, s, lo, cnt, toff, nA, permB, cA, cB, kA, nK, ta, tb,
+                     key);
+  SHT_HIP(hipGetLastError());
+
+  // sort tasks by (C id, key)
+  uint64_t* key_s = S.get<uint64_t>(nt);
+  int64_t* iotaT = S.get<int64_t>(nt);
+  int64_t* order = S.get<int64_t>(nt);
+  hipLaunchKernelGGL(k_iota, dim3(blocks_for(nt)), dim3(256), 0, s, iotaT, nt);
+  const int cbits = bits_for(static_cast<uint64_t>(C->nblocks_total) * static_cast<uint64_t>(nK));
+  tmp = 0;
+  SHT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key_s, iotaT, order, static_cast<int>(nt), 0, cbits, s));
+  SHT_HIP(hipcub::DeviceRadixSort::SortPairs(S.bytes(tmp), tmp, key, key_s, iotaT, order, static_cast<int>(nt), 0,
+                                             cbits, s));
+  int64_t* ta_s = S.get<int64_t>(nt);
+  int64_t* tb_s = S.get<int64_t>(nt);
+  int64_t* cid_s = S.get<int64_t>(nt);
+  int64_t* kk_s = S.get<int64_t>(nt);
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(nt)), dim3(256), 0, s, order, nt, ta, tb, key_s, nK, ta_s, tb_s, cid_s,
+                     kk_s);
+  SHT_HIP(hipGetLastError());
+
+  int64_t* tc = S.get<int64_t>(nt);
+  if (!keep) {
+    int64_t* flag = S.get<int64_t>(nt);
+    hipLaunchKernelGGL(k_headflags, dim3(blocks_for(nt)), dim3(256), 0, s, cid_s, nt, flag);
+    tmp = 0;
+    SHT_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, flag, tc, static_cast<int>(nt), s));
+    SHT_HIP(hipcub::DeviceScan::InclusiveSum(S.bytes(tmp), tmp, flag, tc, static_cast<int>(nt), s));
+    int64_t ncb = 0;
+    SHT_HIP(hipMemcpy(&ncb, tc + nt - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t* uids = S.get<int64_t>(ncb);
+    hipLaunchKernelGGL(k_unique, dim3(blocks_for(nt)), dim3(256), 0, s, cid_s, flag, tc, nt, uids);
+    SHT_HIP(hipGetLastError());
+    std::vector<int64_t> ids(static_cast<size_t>(ncb));
+    SHT_HIP(hipMemcpy(ids.data(), uids, ncb * sizeof(int64_t), hipMemcpyDeviceToHost));
+    tensor_reserve_sorted(C, std::move(ids));
+  } else {
+    char* kf = S.get<char>(nt);
+    int64_t* tcl = S.get<int64_t>(nt);
+    hipLaunchKernelGGL(k_lookup, dim3(blocks_for(nt)), dim3(256), 0, s, cid_s, nt, C->d_indices,
+                       static_cast<int64_t>(C->indices.size()), tcl, kf);
+    SHT_HIP(hipGetLastError());
+    // compact the four task arrays by the keep flags
+    int* nsel = S.get<int>(1);
+    int64_t* bufs_in[4] = {ta_s, tb_s, tcl, kk_s};
+    int64_t* bufs_out[4] = {S.get<int64_t>(nt), S.get<int64_t>(nt), tc, S.get<int64_t>(nt)};
+    for (int q = 0; q < 4; ++q) {
+      tmp = 0;
+      SHT_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp, bufs_in[q], kf, bufs_out[q], nsel, static_cast<int>(nt), s));
+      SHT_HIP(hipcub::DeviceSelect::Flagged(S.bytes(tmp), tmp, bufs_in[q], kf, bufs_out[q], nsel, static_cast<int>(nt),
+                                            s));
+    }
+    int ns = 0;
+    SHT_HIP(hipMemcpy(&ns, nsel, sizeof(int), hipMemcpyDeviceToHost));
+    nt = ns;
+    ta_s = bufs_out[0];
+    tb_s = bufs_out[1];
+    kk_s = bufs_out[3];
+  }
+  P->ntasks = nt;
+  P->ncb = static_cast<int64_t>(C->indices.size());
+
+  // keep the task list
+  P->d_ta = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(nt, 1) * sizeof(int64_t)));
+  P->d_tb = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(nt, 1) * sizeof(int64_t)));
+  P->d_tc = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(nt, 1) * sizeof(int64_t)));
+  P->d_kk = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(nt, 1) * sizeof(int64_t)));
+  if (nt) {
+    SHT_HIP(hipMemcpyAsync(P->d_ta, ta_s, nt * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    SHT_HIP(hipMemcpyAsync(P->d_tb, tb_s, nt * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    SHT_HIP(hipMemcpyAsync(P->d_tc, tc, nt * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    SHT_HIP(hipMemcpyAsync(P->d_kk, kk_s, nt * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+  }
+  SHT_HIP(hipStreamSynchronize(s));
+}
+
+void build_schedule(sht_plan* P)
+{
+  Arena& arena = P->ctx->arena;
+  const int64_t nt = P->ntasks, ncb = P->ncb;
+  const ContractMaps& m = P->maps;
+  hipStream_t s = nullptr;
+  Scratch S(arena);
+  P->d_tasks = static_cast<TaskRec*>(arena.allocate(std::max<int64_t>(nt, 1) * sizeof(TaskRec)));
+  P->d_cblk = static_cast<CBlkRec*>(arena.allocate(std::max<int64_t>(ncb, 1) * sizeof(CBlkRec)));
+  int64_t* cstart = S.get<int64_t>(ncb + 1);
+  int64_t* work = S.get<int64_t>(ncb);
+  if (nt)
+    hipLaunchKernelGGL(k_task_recs, dim3(blocks_for(nt)), dim3(256), 0, s, P->d_ta, P->d_tb, nt, P->A->desc(),
+                       P->B->desc(), axis_list(m.a_col), P->d_tasks);
+  hipLaunchKernelGGL(k_cstart, dim3(blocks_for(ncb + 1)), dim3(256), 0, s, P->d_tc, nt, ncb, cstart);
+  if (ncb)
+    hipLaunchKernelGGL(k_cblk_recs, dim3(blocks_for(ncb)), dim3(256), 0, s, P->C->desc(), axis_list(m.c_row),
+                       axis_list(m.c_col), cstart, P->d_tasks, P->d_cblk, work);
+  SHT_HIP(hipGetLastError());
+  std::vector<CBlkRec> cb(static_cast<size_t>(ncb));
+  std::vector<int64_t> w(static_cast<size_t>(ncb));
+  if (ncb) {
+    SHT_HIP(hipMemcpy(cb.data(), P->d_cblk, ncb * sizeof(CBlkRec), hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(w.data(), work, ncb * sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+  // useful flops
+  double fl = 0.0;
+  for (int64_t c = 0; c < ncb; ++c) fl += 2.0 * cb[c].m * static_cast<double>(cb[c].n) * static_cast<double>(w[c]);
+  P->flops = fl;
+  const double es = 8.0;
+  P->bytes_in = es * (static_cast<double>(P->A->nelem) + static_cast<double>(P->B->nelem) +
+                      static_cast<double>(P->C->nelem));
+  P->bytes_c = es * static_cast<double>(P->C->nelem);
+
+  // tiles per kernel class, each class ordered by descending cost
+  struct T {
+    TileRec r;
+    double cost;
+  };
+  std::vector<std::vector<T>> per(static_cast<int>(KClass::Count));
+  const bool exact = P->prec == SHT_PREC_FP64_EXACT;
+  for (int64_t c = 0; c < ncb; ++c) {
+    const int M = cb[c].m, N = cb[c].n;
+    KClass k;
+    if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
+    else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : KClass::F32x32;
+    else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : KClass::Bf16x32;
+    else k = N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
+    const int rows = kclass_rows(k), cols = kclass_cols(k);
+    for (int m0 = 0; m0 < M; m0 += rows)
+      for (int n0 = 0; n0 < N; n0 += cols) {
+        const double cost = static_cast<double>(std::min(rows, M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
+        per[static_cast<int>(k)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost});
+      }
+  }
+  std::vector<TileRec> all;
+  P->launches.clear();
+  for (int k = 0; k < static_cast<int>(KClass::Count); ++k) {
+    auto& v = per[k];
+    if (v.empty()) continue;
+    std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
+    P->launches.push_back({static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size())});
+    for (const T& t : v) all.push_back(t.r);
+  }
+  if (!all.empty()) {
+    P->d_tiles = static_cast<TileRec*>(arena.allocate(all.size() * sizeof(TileRec)));
+    SHT_HIP(hipMemcpy(P->d_tiles, all.data(), all.size() * sizeof(TileRec), hipMemcpyHostToDevice));
+  }
+}
+
+void setup_staging(sht_plan* P)
+{
+  const ContractMaps& m = P->maps;
+  P->permA = !canonical(m.a_row, m.a_col);
+  P->permB = !canonical(m.b_row, m.b_col);
+  P->permC = !canonical(m.c_row, m.c_col);
+  auto order = [](const std::vector<int>& a, const std::vector<int>& b, int* out) {
+    int i = 0;
+    for (int x : a) out[i++] = x;
+    for (int x : b) out[i++] = x;
+  };
+  order(m.a_row, m.a_col, P->orderA);
+  order(m.b_row, m.b_col, P->orderB);
+  order(m.c_row, m.c_col, P->orderC);
+  Arena& arena = P->ctx->arena;
+  if (P->permA) P->d_Ap = arena.allocate(std::max<int64_t>(P->A->nelem, 1) * 8);
+  if (P->permB) P->d_Bp = arena.allocate(std::max<int64_t>(P->B->nelem, 1) * 8);
+  if (P->permC) P->d_Cp = arena.allocate(std::max<int64_t>(P->C->nelem, 1) * 8);
+}
+
+void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<hipEvent_t>* ev = nullptr)
+{
+  auto zero_bytes = [](double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    return u == 0;
+  };
+  const bool az = zero_bytes(alpha), bz = zero_bytes(beta);
+  const void* Ad = P->A->d_data;
+  const void* Bd = P->B->d_data;
+  void* Cd = P->C->d_data;
+  auto perm = [&](const sht_tensor* t, const void* in, void* out, const int* order, int inverse) {
+    PermArgs pa{};
+    pa.d = t->desc();
+    pa.in = in;
+    pa.out = out;
+    for (int i = 0; i < t->N; ++i) pa.order[i] = order[i];
+    pa.inverse = inverse;
+    launch_permute(pa, true, s);
+  };
+  if (P->permA && !az) {
+    perm(P->A, Ad, P->d_Ap, P->orderA, 0);
+    Ad = P->d_Ap;
+  }
+  if (P->permB && !az) {
+    perm(P->B, Bd, P->d_Bp, P->orderB, 0);
+    Bd = P->d_Bp;
+  }
+  if (P->permC) {
+    if (!bz) perm(P->C, Cd, P->d_Cp, P->orderC, 0);
+    Cd = P->d_Cp;
+  }
+  ExecArgs a{};
+  a.A = Ad;
+  a.B = Bd;
+  a.C = Cd;
+  a.cblk = P->d_cblk;
+  a.tasks = P->d_tasks;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.alpha_zero = az ? 1 : 0;
+  a.beta_zero = bz ? 1 : 0;
+  a.alpha_one = alpha == 1.0 ? 1 : 0;
+  for (size_t i = 0; i < P->launches.size(); ++i) {
+    const auto& L = P->launches[i];
+    a.tiles = P->d_tiles + L.first;
+    a.ntiles = L.count;
+    if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i], s));
+    launch_exec(L.k, a, s);
+    if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i + 1], s));
+  }
+  if (P->permC) perm(P->C, P->d_Cp, P->C->d_data, P->orderC, 1);
+}
+
+}  // namespace
+
+}  // namespace sht
+
+using sht::Error;
+using sht::guard;
+
+extern "C" {
+
+sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, const sht_tensor* B, sht_tensor* C,
+                           sht_precision prec, sht_plan** out)
+{
+  return guard([&] {
+    SHT_REQUIRE(ctx && expr && out, SHT_ERR_INVALID, "null argument");
+    sht::check_tensors(A, B, C, ctx);
+    SHT_REQUIRE(prec >= SHT_PREC_FP64 && prec <= SHT_PREC_BF16X3, SHT_ERR_INVALID, "unknown precision");
+    auto P = std::make_unique<sht_plan>();
+    P->ctx = ctx;
+    P->expr = expr;
+    P->A = A;
+    P->B = B;
+    P->C = C;
+    P->prec = prec;
+    // block-level analysis: the expression over block counts
+    P->maps = sht::analyse_expression(P->expr, A->nblk, B->nblk, C->nblk);
+    const sht::ContractMaps& m = P->maps;
+    for (size_t r = 0; r < m.a_row.size(); ++r)
+      SHT_REQUIRE(A->bs[m.a_row[r]] == C->bs[m.c_row[r]], SHT_ERR_INVALID, "Block sizes of A and C do not match");
+    for (size_t r = 0; r < m.b_col.size(); ++r)
+      SHT_REQUIRE(B->bs[m.b_col[r]] == C->bs[m.c_col[r]], SHT_ERR_INVALID, "Block sizes of B and C do not match");
+    for (size_t r = 0; r < m.a_col.size(); ++r)
+      SHT_REQUIRE(A->bs[m.a_col[r]] == B->bs[m.b_row[r]], SHT_ERR_INVALID, "Block sizes of A and B do not match");
+    try {
+      sht::build_tasks(P.get());
+      sht::build_schedule(P.get());
+      sht::setup_staging(P.get());
+    } catch (...) {
+      P->release();
+      throw;
+    }
+    *out = P.release();
+  });
+}
+
+sht_status sht_plan_execute(sht_plan* P, double alpha, double beta, void* stream)
+{
+  return guard([&] {
+    SHT_REQUIRE(P, SHT_ERR_INVALID, "null plan");
+    SHT_REQUIRE(P->C->reserved && static_cast<int64_t>(P->C->indices.size()) == P->ncb, SHT_ERR_STATE,
+                "C pattern changed since the plan was built");
+    sht::enqueue(P, alpha, beta, sht::as_stream(stream));
+  });
+}
+
+sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* stream, int* kclass_out,
+                                  float* ms_out, int cap, int* nlaunch)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && nlaunch, SHT_ERR_INVALID, "null argument");
+    const size_t n = P->launches.size();
+    std::vector<hipEvent_t> ev(2 * n);
+    for (auto& e : ev) SHT_HIP(hipEventCreate(&e));
+    sht_status st = SHT_OK;
+    std::string msg;
+    try {
+      hipStream_t s = sht::as_stream(stream);
+      sht::enqueue(P, alpha, beta, s, &ev);
+      SHT_HIP(hipStreamSynchronize(s));
+      for (size_t i = 0; i < n && static_cast<int>(i) < cap; ++i) {
+        float ms = 0.f;
+        SHT_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        if (ms_out) ms_out[i] = ms;
+        if (kclass_out) kclass_out[i] = static_cast<int>(P->launches[i].k);
+      }
+      *nlaunch = static_cast<int>(n);
+    } catch (const Error& e) {
+      st = e.code;
+      msg = e.what();
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (st != SHT_OK) throw Error(st, msg);
+  });
+}
+
+const char* sht_kclass_name(int k)
+{
+  static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
+                                "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
+  if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
+  return names[k];
+}
+
+sht_status sht_plan_graph_replay(sht_plan* P, double alpha, double beta, int replays, void* stream)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && replays >= 0, SHT_ERR_INVALID, "bad argument");
+    hipStream_t s = sht::as_stream(stream);
+    SHT_REQUIRE(s != nullptr, SHT_ERR_INVALID, "graph replay needs a created stream");
+    if (!P->graph || P->graph_alpha != alpha || P->graph_beta != beta) {
+      if (P->graph) SHT_HIP(hipGraphExecDestroy(P->graph));
+      if (P->graph_def) SHT_HIP(hipGraphDestroy(P->graph_def));
+      P->graph = nullptr;
+      P->graph_def = nullptr;
+      SHT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        sht::enqueue(P, alpha, beta, s);
+      } catch (...) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      SHT_HIP(hipStreamEndCapture(s, &P->graph_def));
+      SHT_HIP(hipGraphInstantiate(&P->graph, P->graph_def, nullptr, nullptr, 0));
+      P->graph_alpha = alpha;
+      P->graph_beta = beta;
+    }
+    for (int r = 0; r < replays; ++r) SHT_HIP(hipGraphLaunch(P->graph, s));
+  });
+}
+
+sht_status sht_plan_tasks(sht_plan* P, const int64_t** ta, const int64_t** tb, const int64_t** tc,
+                          const int64_t** kkey, int64_t* ntasks)
+{
+  return guard([&] {
+    SHT_REQUIRE(P, SHT_ERR_INVALID, "null plan");
+    if (!P->host_tasks) {
+      const int64_t n = P->ntasks;
+      P->h_ta.resize(n);
+      P->h_tb.resize(n);
+      P->h_tc.resize(n);
+      P->h_kk.resize(n);
+      if (n) {
+        SHT_HIP(hipMemcpy(P->h_ta.data(), P->d_ta, n * 8, hipMemcpyDeviceToHost));
+        SHT_HIP(hipMemcpy(P->h_tb.data(), P->d_tb, n * 8, hipMemcpyDeviceToHost));
+        SHT_HIP(hipMemcpy(P->h_tc.data(), P->d_tc, n * 8, hipMemcpyDeviceToHost));
+        SHT_HIP(hipMemcpy(P->h_kk.data(), P->d_kk, n * 8, hipMemcpyDeviceToHost));
+      }
+      P->host_tasks = true;
+    }
+    if (ta) *ta = P->h_ta.data();
+    if (tb) *tb = P->h_tb.data();
+    if (tc) *tc = P->h_tc.data();
+    if (kkey) *kkey = P->h_kk.data();
+    if (ntasks) *ntasks = P->ntasks;
+  });
+}
+
+sht_status sht_plan_stats(const sht_plan* P, double* flops, double* bytes, int64_t* ntasks, int64_t* ncblocks)
+{
+  return guard([&] {
+    SHT_REQUIRE(P, SHT_ERR_INVALID, "null plan");
+    if (flops) *flops = P->flops;
+    if (bytes) *bytes = P->bytes_in;
+    if (ntasks) *ntasks = P->ntasks;
+    if (ncblocks) *ncblocks = P->ncb;
+  });
+}
+
+sht_status sht_plan_destroy(sht_plan* P)
+{
+  return guard([&] {
+    if (!P) return;
+    P->release();
+    delete P;
+  });
+}
+
+sht_status sht_contract(sht_ctx* ctx, const char* expr, double alpha, const sht_tensor* A, const sht_tensor* B,
+                        double beta, sht_tensor* C, sht_precision prec, void* stream)
+{
+  return guard([&] {
+    sht_plan* P = nullptr;
+    sht_status st = sht_plan_create(ctx, expr, A, B, C, prec, &P);
+    if (st != SHT_OK) throw Error(st, sht_last_error());
+    st = sht_plan_execute(P, alpha, beta, stream);
+    if (st == SHT_OK) {
+      hipError_t e = hipStreamSynchronize(sht::as_stream(stream));
+      if (e != hipSuccess) {
+        sht_plan_destroy(P);
+        throw Error(SHT_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+      }
+    }
+    const std::string msg = st == SHT_OK ? "" : sht_last_error();
+    sht_plan_destroy(P);
+    if (st != SHT_OK) throw Error(st, msg);
+  });
+}
+
+}  // extern "C"

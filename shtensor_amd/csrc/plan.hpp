@@ -1,0 +1,69 @@
+// Contraction plan: task list, C pattern and execution schedule on the GPU.
+#pragma once
+
+#include "internal.hpp"
+
+namespace sht {
+
+// One C-stationary tile: rows [m0, m0+rows) and columns [n0, n0+cols) of
+// C block `cb`; the kernel class fixes rows/cols.
+struct TileRec {
+  int32_t cb;
+  int32_t m0;
+  int32_t n0;
+  int32_t pad;
+};
+
+// Per C block: element offset, shape and task range (tasks sorted by K key).
+struct CBlkRec {
+  int64_t c_off;
+  int32_t m, n;
+  int32_t t0, t1;
+};
+
+// Per task: element offsets of the A and B blocks and the contracted length.
+struct TaskRec {
+  int64_t a_off;
+  int64_t b_off;
+  int32_t k;
+  int32_t pad;
+};
+
+struct ExecArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  const TileRec* tiles;
+  int64_t ntiles;
+  const CBlkRec* cblk;
+  const TaskRec* tasks;
+  double alpha, beta;
+  int alpha_zero, beta_zero, alpha_one;
+};
+
+// kernel classes of the C-stationary launches
+enum class KClass : int {
+  Exact8 = 0, Exact16, Exact32,      // fma chain, 128 rows x {8,16,32} cols
+  Mfma16, Mfma32,                    // FP64 MFMA, 64 rows x {16,32} cols
+  F32x16, F32x32,                    // FP32 MFMA, FP64 cross-task
+  Bf16x16, Bf16x32,                  // split-BF16x3 MFMA, FP64 cross-task
+  Count
+};
+
+int kclass_rows(KClass k);
+int kclass_cols(KClass k);
+
+void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
+
+// Block permutation (Utils::reshape semantics per block, Utils.h:315-359):
+// out block = in block with axes reordered by `order`.
+struct PermArgs {
+  TensorDesc d;
+  const void* in;
+  void* out;
+  int order[kMaxModes];
+  int inverse;  // 0: src layout -> canonical, 1: canonical -> src layout
+};
+void launch_permute(const PermArgs& p, bool f64, hipStream_t s);
+
+}  // namespace sht

@@ -1,0 +1,199 @@
+"""GPU parity of the block-sparse contraction with the oracle.
+
+Task list and C block pattern: bit-exact.  Values: FP64_EXACT bitwise vs the
+oracle's fma chain; FP64 (MFMA) within 1e-12 normwise; FP32 within 1e-6;
+split-BF16x3 within 1e-4; integer-valued operands bitwise in every form (all
+products and partial sums are exact)."""
+import numpy as np
+import pytest
+
+import shtensor_amd as sht
+from oracle import oracle as orc
+from gpu_util import make_case, normwise, to_device
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("ijk,kl->ijl", "ijk", "kl", "ijl"),
+    ("ijkl,klm->ijm", "ijkl", "klm", "ijm"),
+    ("ik,kj->ij", "ik", "kj", "ij"),
+    ("kil,ljk->ij", "kil", "ljk", "ij"),
+    ("ik,kjl->ilj", "ik", "kjl", "ilj"),
+    ("akbl,lcka->bc", "akbl", "lcka", "bc"),
+    ("ijk,km->ijm", "ijk", "km", "ijm"),
+]
+PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64, "fp32": sht.PREC_FP32,
+         "bf16x3": sht.PREC_BF16X3}
+TOL = {"fp64_exact": 0.0, "fp64": 1e-12, "fp32": 1e-6, "bf16x3": 1e-4}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sht.Context(4 << 30)
+    yield c
+    c.close()
+
+
+def oracle_result(expr, A, B, Cp, alpha=1.0, beta=0.0, c0=None, mode=0):
+    t, kk, cids = orc.tasks(expr, A, B, Cp)
+    R = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids if len(Cp.indices) == 0 else Cp.indices)
+    if c0 is not None:
+        R.data[:] = c0
+    orc.contract(expr, alpha, A, B, beta, R, mode=mode)
+    return t, kk, cids, R
+
+
+@pytest.mark.parametrize("prec", list(PRECS))
+@pytest.mark.parametrize("expr,ea,eb,ec", CASES)
+def test_contract_vs_oracle(ctx, expr, ea, eb, ec, prec):
+    print("contract", expr, prec, flush=True)
+    rng = np.random.default_rng(len(expr) * 7 + len(prec))
+    A, B, Cp = make_case(rng, ea, eb, ec, p=0.6, bmin=2, bmax=9)
+    t, kk, cids, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
+    got = plan.tasks()
+    assert np.array_equal(got["a"], t[:, 0]) and np.array_equal(got["b"], t[:, 1])
+    assert np.array_equal(got["c"], t[:, 2]) and np.array_equal(got["key"], kk)
+    assert np.array_equal(dC.get_sparse_info()["indices"], cids)
+    assert np.array_equal(dC.get_sparse_info()["offsets"], R.offsets)
+    plan.execute(1.0, 0.0)
+    sht.call("sht_device_synchronize")
+    out = dC.download()
+    if prec == "fp64_exact":
+        assert np.array_equal(out, R.data)
+    else:
+        assert normwise(out, R.data) <= TOL[prec], normwise(out, R.data)
+    plan.close()
+
+
+@pytest.mark.parametrize("prec", list(PRECS))
+@pytest.mark.parametrize("expr,ea,eb,ec", CASES[:4])
+def test_contract_integer_exact_every_form(ctx, expr, ea, eb, ec, prec):
+    rng = np.random.default_rng(99)
+    A, B, Cp = make_case(rng, ea, eb, ec, p=0.7, integer=True, bmin=2, bmax=7)
+    _, _, _, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    sht.contract(ctx, expr, 1.0, dA, dB, 0.0, dC, PRECS[prec])
+    assert np.array_equal(dC.download(), R.data)
+
+
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64"])
+def test_alpha_beta_kept_pattern(ctx, prec):
+    rng = np.random.default_rng(7)
+    expr = "ijkl,klm->ijm"
+    A, B, Cp = make_case(rng, "ijkl", "klm", "ijm", p=0.6, bmin=2, bmax=9)
+    _, _, cids = orc.tasks(expr, A, B, Cp)
+    keep = cids[::2]
+    Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(keep)
+    c0 = rng.uniform(-1, 1, len(Ck.data))
+    Ck.data[:] = c0
+    _, _, _, R = oracle_result(expr, A, B, Ck, alpha=0.5, beta=-1.25, c0=c0)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = to_device(ctx, Ck)
+    sht.contract(ctx, expr, 0.5, dA, dB, -1.25, dC, PRECS[prec])
+    assert np.array_equal(dC.get_sparse_info()["indices"], keep)
+    out = dC.download()
+    if prec == "fp64_exact":
+        assert np.array_equal(out, R.data)
+    else:
+        assert normwise(out, R.data) <= 1e-12
+
+
+def test_alpha_zero_scales_only(ctx):
+    rng = np.random.default_rng(8)
+    A, B, Cp = make_case(rng, "ik", "kj", "ij", p=0.8)
+    _, _, cids = orc.tasks("ik,kj->ij", A, B, Cp)
+    Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+    Ck.data[:] = rng.uniform(-1, 1, len(Ck.data))
+    dA, dB, dC = to_device(ctx, A), to_device(ctx, B), to_device(ctx, Ck)
+    sht.contract(ctx, "ik,kj->ij", 0.0, dA, dB, 3.0, dC, sht.PREC_FP64)
+    assert np.array_equal(dC.download(), 3.0 * Ck.data)
+
+
+def test_empty_operand(ctx):
+    A = orc.BlockSparse([[2, 3], [4]])
+    A.reserve_ids([])
+    B = orc.BlockSparse([[4], [5, 1]])
+    B.reserve_ids([0, 1])
+    B.data[:] = 1.0
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, [[2, 3], [5, 1]])
+    sht.contract(ctx, "ik,kj->ij", 1.0, dA, dB, 0.0, dC)
+    assert dC.get_nb_nzblocks_local() == 0
+
+
+def test_expression_and_block_size_errors(ctx):
+    A = sht.Tensor(ctx, [[2, 3], [4]])
+    A.reserve_all()
+    B = sht.Tensor(ctx, [[4], [5]])
+    B.reserve_all()
+    C = sht.Tensor(ctx, [[2, 3], [5]])
+    with pytest.raises(sht.ShtError, match="Duplicate tensor indices"):
+        sht.contract(ctx, "ii,kj->ij", 1.0, A, B, 0.0, C)
+    B2 = sht.Tensor(ctx, [[3, 1], [5]])
+    B2.reserve_all()
+    with pytest.raises(sht.ShtError, match="Sizes for dimensions|Block sizes"):
+        sht.contract(ctx, "ik,kj->ij", 1.0, A, B2, 0.0, C)
+    B3 = sht.Tensor(ctx, [[4], [5, 2]])
+    B3.reserve_all()
+    with pytest.raises(sht.ShtError):
+        sht.contract(ctx, "ik,kj->ij", 1.0, A, B3, 0.0, C)
+
+
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "bf16x3"])
+def test_graph_replay_matches_eager(ctx, prec):
+    rng = np.random.default_rng(12)
+    expr = "ijk,kl->ijl"
+    A, B, Cp = make_case(rng, "ijk", "kl", "ijl", p=0.6, bmin=3, bmax=12)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
+    plan.execute(1.0, 0.0)
+    sht.call("sht_device_synchronize")
+    eager = dC.download()
+    s = sht.Stream()
+    for rep in range(3):
+        dC.upload(np.full(dC.nelem(), 7.0))
+        plan.graph_replay(1.0, 0.0, 2, s)
+        s.synchronize()
+        assert np.array_equal(dC.download(), eager), rep
+    s.close()
+    plan.close()
+
+
+def test_config0_full_size(ctx):
+    """ijk,kl->ijl, every mode 256 in 16 blocks of 16, p=0.3 (seeded)."""
+    seed = 1000
+    bs = [[16] * 16] * 3
+    bsB = [[16] * 16] * 2
+    A = orc.BlockSparse(bs).reserve_ids(orc.gen_presence(seed, 0, [16, 16, 16], 0.3))
+    B = orc.BlockSparse(bsB).reserve_ids(orc.gen_presence(seed, 1, [16, 16], 0.3))
+    orc.gen_values(seed, 0, A, 0.0)
+    orc.gen_values(seed, 1, B, 0.0)
+    Cp = orc.BlockSparse(bs)
+    t, kk, cids, R = oracle_result("ijk,kl->ijl", A, B, Cp)
+    dA = sht.Tensor(ctx, bs)
+    dA.reserve_random(seed, 0, 0.3)
+    dA.fill_random(seed, 0)
+    dB = sht.Tensor(ctx, bsB)
+    dB.reserve_random(seed, 1, 0.3)
+    dB.fill_random(seed, 1)
+    for prec in ("fp64_exact", "fp64"):
+        dC = sht.Tensor(ctx, bs)
+        plan = sht.Plan(ctx, "ijk,kl->ijl", dA, dB, dC, PRECS[prec])
+        got = plan.tasks()
+        assert len(got["a"]) == len(t)
+        assert np.array_equal(got["a"], t[:, 0]) and np.array_equal(got["b"], t[:, 1])
+        assert np.array_equal(dC.get_sparse_info()["indices"], cids)
+        plan.execute()
+        sht.call("sht_device_synchronize")
+        out = dC.download()
+        if prec == "fp64_exact":
+            assert np.array_equal(out, R.data)
+        else:
+            assert normwise(out, R.data) <= 1e-12
+        plan.close()
+        dC.close()

@@ -1,0 +1,119 @@
+"""GPU parity of the block-product Kernel (Kernel<T>::call replacement) with
+the oracle's XMM chain: bitwise for every expression of
+test/test_kernel.cpp, both dtypes, generic and hipRTC-specialised paths."""
+import os
+
+import numpy as np
+import pytest
+
+import shtensor_amd as sht
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("ik, kj -> ij", [8, 5], [5, 6], [8, 6]),
+    ("jik, kmj -> im", [6, 5, 8], [8, 4, 6], [5, 4]),
+    ("ikl, lkmj -> mij", [6, 5, 7], [7, 5, 6, 6], [6, 6, 6]),
+    ("ijml, lmk -> jki", [5, 6, 8, 9], [9, 8, 7], [6, 7, 5]),
+    ("ij, jk -> ik", [2, 2], [2, 2], [2, 2]),
+    ("ij, jk -> ik", [5, 5], [5, 5], [5, 5]),
+    ("ij, jk -> ik", [8, 8], [8, 8], [8, 8]),
+    ("ij, jk -> ik", [10, 10], [10, 10], [10, 10]),
+    ("ij, jk -> ki", [4, 4], [4, 4], [4, 4]),
+    ("ikl, ljk -> ij", [8, 8, 8], [8, 8, 8], [8, 8]),
+    ("kil, ljk -> ij", [8, 8, 8], [8, 8, 8], [8, 8]),
+    ("ikl, ljk -> ji", [5, 5, 5], [5, 5, 5], [5, 5]),
+    ("ik, kj -> ij", [16, 16], [16, 16], [16, 16]),
+    ("ik,kjl->ilj", [3, 4], [4, 5, 2], [3, 2, 5]),
+    ("ijk,kl->ijl", [16, 16, 16], [16, 16], [16, 16, 16]),
+]
+
+
+def run_gpu(expr, sa, sb, sc, alpha, beta, a, b, c, nb, dtype):
+    k = sht.Kernel(expr, sa, sb, sc, alpha, beta, sht.XMM, dtype)
+    da, db, dc = sht.DeviceArray(a), sht.DeviceArray(b), sht.DeviceArray(c)
+    k.call(da, db, dc, nb)
+    sht.call("sht_device_synchronize")
+    out = dc.numpy()
+    k.close()
+    return out
+
+
+@pytest.mark.parametrize("expr,sa,sb,sc", CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.75, 1.5)])
+def test_kernel_bitwise_vs_oracle(expr, sa, sb, sc, dtype, alpha, beta):
+    print("kernel", expr, sa, sb, sc, np.dtype(dtype).name, alpha, beta, flush=True)
+    rng = np.random.default_rng(abs(hash((expr, tuple(sa)))) % 2 ** 32)
+    nb = 7
+    a = rng.uniform(-1, 1, nb * int(np.prod(sa))).astype(dtype)
+    b = rng.uniform(-1, 1, nb * int(np.prod(sb))).astype(dtype)
+    c = rng.uniform(-1, 1, nb * int(np.prod(sc))).astype(dtype)
+    want = c.copy()
+    orc.kernel_xmm(expr, sa, sb, sc, alpha, beta, a, b, want, nb)
+    got = run_gpu(expr, sa, sb, sc, alpha, beta, a, b, c, nb, dtype)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("expr,sa,sb,sc", CASES[:5])
+def test_kernel_generic_path_bitwise(expr, sa, sb, sc, monkeypatch):
+    monkeypatch.setenv("SHT_JIT", "0")
+    rng = np.random.default_rng(5)
+    nb = 3
+    a = rng.uniform(-1, 1, nb * int(np.prod(sa)))
+    b = rng.uniform(-1, 1, nb * int(np.prod(sb)))
+    c = rng.uniform(-1, 1, nb * int(np.prod(sc)))
+    want = c.copy()
+    orc.kernel_xmm(expr, sa, sb, sc, 1.0, 1.0, a, b, want, nb)
+    k = sht.Kernel(expr, sa, sb, sc, 1.0, 1.0)
+    assert "generic" in k.get_info()
+    da, db, dc = sht.DeviceArray(a), sht.DeviceArray(b), sht.DeviceArray(c)
+    k.call(da, db, dc, nb)
+    sht.call("sht_device_synchronize")
+    assert np.array_equal(dc.numpy(), want)
+
+
+def test_kernel_beta_identity():
+    # test/test_kernel.cpp:241-285 (alpha = 0, beta = 2)
+    sa, sb, sc = [5, 6, 8, 9], [9, 8, 7], [6, 7, 5]
+    rng = np.random.default_rng(1)
+    a = rng.uniform(-1, 1, int(np.prod(sa))).astype(np.float32)
+    b = rng.uniform(-1, 1, int(np.prod(sb))).astype(np.float32)
+    c = rng.uniform(-1, 1, int(np.prod(sc))).astype(np.float32)
+    got = run_gpu("ijml, lmk -> jki", sa, sb, sc, 0.0, 2.0, a, b, c, 1, np.float32)
+    assert np.array_equal(got, 2 * c)
+    assert abs(got.sum() - 2 * c.sum()) < 1e-6 * max(1.0, abs(c.sum()))
+
+
+def test_kernel_nb_ops_zero_noop_and_many_ops():
+    rng = np.random.default_rng(2)
+    sa, sb, sc = [16, 16, 16], [16, 16], [16, 16, 16]
+    nb = 300
+    a = rng.uniform(-1, 1, nb * 4096)
+    b = rng.uniform(-1, 1, nb * 256)
+    c = rng.uniform(-1, 1, nb * 4096)
+    k = sht.Kernel("ijk,kl->ijl", sa, sb, sc, 1.0, 0.0)
+    da, db, dc = sht.DeviceArray(a), sht.DeviceArray(b), sht.DeviceArray(c)
+    k.call(da, db, dc, 0)
+    sht.call("sht_device_synchronize")
+    assert np.array_equal(dc.numpy(), c)
+    k.call(da, db, dc, nb)
+    sht.call("sht_device_synchronize")
+    want = c.copy()
+    orc.kernel_xmm("ijk,kl->ijl", sa, sb, sc, 1.0, 0.0, a, b, want, nb)
+    assert np.array_equal(dc.numpy(), want)
+
+
+def test_kernel_errors_and_info():
+    with pytest.raises(sht.ShtError, match="Duplicate tensor indices"):
+        sht.Kernel("ii,ik->ik", [2, 2], [2, 2], [2, 2])
+    with pytest.raises(sht.ShtError, match="Sizes for dimensions 1 in tensor A"):
+        sht.Kernel("ij,jk->ik", [2, 3], [2, 4], [2, 4])
+    with pytest.raises(sht.ShtError, match="Unknown kernel type"):
+        sht.Kernel("ij,jk->ik", [2, 3], [3, 4], [2, 4], kernel_type=7)
+    k = sht.Kernel("jik, kmj -> im", [6, 5, 8], [8, 4, 6], [5, 4], kernel_type=sht.LAPACK)
+    info = k.get_info()
+    assert "Expression: jik, kmj -> im" in info
+    assert "map_row: 1\nmap_col: 0,2\n" in info
+    assert "scatter row: 0,6,12,18,24\n" in info
