@@ -23,7 +23,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6e12, "fp64_exact": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+PEAK = {"fp64": 78.6e12, "fp64_cs": 78.6e12, "fp64_as": 78.6e12, "fp64_exact": 78.6e12, "fp32": 157.3e12,
+        "bf16x3": 2.5e15}
 HBM_PEAK = 8.0e12
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
 
@@ -135,7 +136,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--length", type=int, default=None, help="override every mode length (smaller runs)")
-    ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3")
+    ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3",
+                    help="fp64 (autotuned schedule), fp64_cs, fp64_as, fp64_exact, fp32, bf16x3")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
@@ -150,8 +152,8 @@ def main():
     import shtensor_amd as sht
 
     t0 = time.time()
-    precs = {"fp64": sht.PREC_FP64, "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32,
-             "bf16x3": sht.PREC_BF16X3}
+    precs = {"fp64": sht.PREC_FP64, "fp64_cs": sht.PREC_FP64, "fp64_as": sht.PREC_FP64,
+             "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
     forms = [f for f in args.forms.split(",") if f]
     arena = int(float(os.environ.get("SHT_BENCH_ARENA_GB", "200")) * (1 << 30))
     ctx = sht.Context(arena)
@@ -166,7 +168,14 @@ def main():
         Cf = sht.Tensor(ctx, Cc.block_sizes)
         tp = time.time()
         plan = sht.Plan(ctx, expr, A, B, Cf, precs[f])
+        tuned = None
+        if f == "fp64":
+            tuned = plan.autotune(1.0, 0.0, reps=2, stream=stream)  # picks the schedule
+        elif f == "fp64_as":
+            plan.set_schedule(1)
         st = plan.stats()
+        st["schedule"] = plan.get_schedule()
+        st["autotune_ms"] = tuned
         log("%s plan: %d tasks, %d C blocks, %.3e flop, %.2fs" % (f, st["ntasks"], st["ncblocks"], st["flops"],
                                                                   time.time() - tp))
         for _ in range(args.warmup):
@@ -264,6 +273,8 @@ def main():
                       "ms_p50": round(1e3 * float(np.percentile(results[f]["times"], 50)), 4),
                       "ms_p99": round(1e3 * float(np.percentile(results[f]["times"], 99)), 4),
                       "launches": results[f]["launches"],
+                      "schedule": results[f]["stats"].get("schedule"),
+                      "autotune_ms": results[f]["stats"].get("autotune_ms"),
                       "parity": parity.get(f)} for f in forms},
         "roofline": roof,
         "cpu_baseline": cpu,

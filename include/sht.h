@@ -162,6 +162,15 @@ sht_status sht_plan_execute_timed(sht_plan* p, double alpha, double beta, void* 
                                   float* ms_out, int cap, int* nlaunch);
 /* Name of a kernel class reported by sht_plan_execute_timed.               */
 const char* sht_kclass_name(int kclass);
+/* Execution schedule: 0 = C-stationary tiles (one tile per workgroup, C
+ * block accumulated in registers over its tasks), 1 = A-stationary row
+ * panels (each A element read once per row chunk; SHT_PREC_FP64 only).     */
+sht_status sht_plan_set_schedule(sht_plan* p, int schedule);
+sht_status sht_plan_get_schedule(const sht_plan* p, int* schedule);
+/* Autotuner: time each applicable schedule (`reps` executions on `stream`)
+ * and keep the fastest; ms_out[schedule] receives each time (may be NULL).
+ * C is overwritten by the trial executions.                                 */
+sht_status sht_plan_autotune(sht_plan* p, double alpha, double beta, int reps, void* stream, float* ms_out);
 /* capture one execution into a hipGraph and replay it `replays` times       */
 sht_status sht_plan_graph_replay(sht_plan* p, double alpha, double beta, int replays, void* stream);
 /* task list (host copies): positions into A, B, C and the contracted-block

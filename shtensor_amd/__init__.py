@@ -351,6 +351,21 @@ class Plan:
              C.byref(n))
         return [(lib().sht_kclass_name(kc[i]).decode(), float(ms[i])) for i in range(min(n.value, 64))]
 
+    def set_schedule(self, schedule):
+        """0: C-stationary tiles, 1: A-stationary row panels (FP64 only)."""
+        call("sht_plan_set_schedule", C.c_void_p(self.ptr), int(schedule))
+
+    def get_schedule(self):
+        v = C.c_int()
+        call("sht_plan_get_schedule", C.c_void_p(self.ptr), C.byref(v))
+        return v.value
+
+    def autotune(self, alpha=1.0, beta=0.0, reps=2, stream=None):
+        """Time every applicable schedule, keep the fastest; {schedule: ms}."""
+        ms = (C.c_float * 2)(-1.0, -1.0)
+        call("sht_plan_autotune", C.c_void_p(self.ptr), float(alpha), float(beta), int(reps), _sptr(stream), ms)
+        return {i: float(ms[i]) for i in range(2) if ms[i] >= 0}
+
     def tasks(self):
         ptrs = [C.POINTER(C.c_int64)() for _ in range(4)]
         n = C.c_int64()

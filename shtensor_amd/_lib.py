@@ -70,6 +70,9 @@ SIGNATURES = {
     "sht_plan_execute": [_P, _D, _D, _P],
     "sht_plan_graph_replay": [_P, _D, _D, _I, _P],
     "sht_plan_execute_timed": [_P, _D, _D, _P, _PI, C.POINTER(C.c_float), _I, _PI],
+    "sht_plan_set_schedule": [_P, _I],
+    "sht_plan_get_schedule": [_P, _PI],
+    "sht_plan_autotune": [_P, _D, _D, _I, _P, C.POINTER(C.c_float)],
     "sht_plan_tasks": [_P, C.POINTER(_PI64), C.POINTER(_PI64), C.POINTER(_PI64), C.POINTER(_PI64), _PI64],
     "sht_plan_stats": [_P, C.POINTER(_D), C.POINTER(_D), _PI64, _PI64],
     "sht_plan_destroy": [_P],

@@ -348,6 +348,91 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
 }
 
 // ---------------------------------------------------------------------------
+// A-stationary FP64 MFMA: one wave per unit (16*RT rows of a block row),
+// partners processed in groups of PG slots of 32 columns; C tiles are read,
+// accumulated over the whole A block and written back (C pre-scaled by beta).
+// ---------------------------------------------------------------------------
+template <int RT, int PG>
+__global__ __launch_bounds__(64) void k_as_mfma64(AsArgs p)
+{
+  const AsUnit u = p.units[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  double* C = static_cast<double*>(p.C);
+  for (int pi = u.pass0; pi < u.pass0 + u.npass; ++pi) {
+    const AsPass ps = p.passes[pi];
+    const double* Ab = A + ps.a_off;
+    const int K = ps.k, M = ps.m;
+    for (int g = 0; g < ps.np; g += PG) {
+      const int ng = min(PG, ps.np - g);
+      AsPartner pt[PG];
+#pragma unroll
+      for (int q = 0; q < PG; ++q) pt[q] = p.partners[ps.p0 + g + (q < ng ? q : 0)];
+      v4d acc[RT][PG][2];
+#pragma unroll
+      for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int i = u.r0 + 16 * rt + 4 * lk + v;
+              const int j = pt[q].n0 + 16 * ct + li;
+              acc[rt][q][ct][v] = (q < ng && i < M && j < pt[q].n) ? C[pt[q].c_off + i + static_cast<int64_t>(j) * M]
+                                                                  : 0.0;
+            }
+      for (int k = 0; k < K; k += 4) {
+        const int kk = k + lk;
+        const bool kin = kk < K;
+        double a[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          const int i = u.r0 + 16 * rt + li;
+          a[rt] = (kin && i < M) ? Ab[i + static_cast<int64_t>(kk) * M] : 0.0;
+          if (!p.alpha_one) a[rt] *= p.alpha;
+        }
+#pragma unroll
+        for (int q = 0; q < PG; ++q) {
+          if (q < ng) {
+            const double* Bb = B + pt[q].b_off;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+              const int j = pt[q].n0 + 16 * ct + li;
+              const double b = (kin && j < pt[q].n) ? Bb[kk + static_cast<int64_t>(j) * K] : 0.0;
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt)
+                acc[rt][q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b, acc[rt][q][ct], 0, 0, 0);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PG; ++q)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int i = u.r0 + 16 * rt + 4 * lk + v;
+              const int j = pt[q].n0 + 16 * ct + li;
+              if (q < ng && i < M && j < pt[q].n) C[pt[q].c_off + i + static_cast<int64_t>(j) * M] = acc[rt][q][ct][v];
+            }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scale(double* c, int64_t n, double beta, int bz)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    c[i] = bz ? 0.0 : c[i] * beta;
+}
+
+// ---------------------------------------------------------------------------
 // block permutation (Utils::reshape per block)
 // ---------------------------------------------------------------------------
 template <class T>
@@ -426,6 +511,27 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Bf16x32: hipLaunchKernelGGL(k_bf16x3<2>, g, b, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
+  SHT_HIP(hipGetLastError());
+}
+
+void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
+{
+  if (a.nunits <= 0) return;
+  const dim3 g(static_cast<unsigned>(a.nunits)), b(kWave);
+  static_assert(kAsRows == 32, "unit rows are 16 * RT");
+  switch (k) {
+    case AsClass::Mfma64: hipLaunchKernelGGL((k_as_mfma64<2, 4>), g, b, 0, s, a); break;
+    default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
+  }
+  SHT_HIP(hipGetLastError());
+}
+
+void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s)
+{
+  if (nelem <= 0) return;
+  const int64_t blocks = (nelem + 255) / 256;
+  const int g = static_cast<int>(blocks < 16384 ? blocks : 16384);
+  hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, s, c, nelem, beta, beta_zero);
   SHT_HIP(hipGetLastError());
 }
 

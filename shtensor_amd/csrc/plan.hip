@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <cmath>
 #include <numeric>
@@ -281,6 +282,13 @@ struct sht_plan {
   double flops = 0.0;
   double bytes_in = 0.0;  // A + B + C write
   double bytes_c = 0.0;   // C read (beta != 0)
+  // A-stationary schedule (built on demand)
+  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary row panels
+  bool as_built = false;
+  sht::AsPass* d_as_pass = nullptr;
+  sht::AsPartner* d_as_part = nullptr;
+  sht::AsUnit* d_as_unit = nullptr;
+  int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   double graph_alpha = 0.0, graph_beta = 0.0;
@@ -290,7 +298,8 @@ struct sht_plan {
     sht::Arena& a = ctx->arena;
     for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
-                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp})
+                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
+                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit)})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);
@@ -545,6 +554,113 @@ void build_schedule(sht_plan* P)
   }
 }
 
+// A-stationary schedule from the task list: tasks regrouped by (block row I,
+// contracted key K, block column J); one pass per A block, one unit per
+// kAsRows-row chunk of each block row, units ordered by descending cost.
+void build_as_schedule(sht_plan* P)
+{
+  if (P->as_built) return;
+  const int64_t nt = P->ntasks;
+  const ContractMaps& m = P->maps;
+  const sht_tensor* A = P->A;
+  const sht_tensor* B = P->B;
+  const sht_tensor* C = P->C;
+  std::vector<int64_t> ta(nt), tb(nt), tc(nt);
+  if (nt) {
+    SHT_HIP(hipMemcpy(ta.data(), P->d_ta, nt * 8, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(tb.data(), P->d_tb, nt * 8, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(tc.data(), P->d_tc, nt * 8, hipMemcpyDeviceToHost));
+  }
+  auto key = [](const sht_tensor* t, int64_t id, const std::vector<int>& axes) {
+    int64_t k = 0, mul = 1;
+    for (int ax : axes) {
+      k += ((id / t->bstride[ax]) % t->nblk[ax]) * mul;
+      mul *= t->nblk[ax];
+    }
+    return k;
+  };
+  auto elems = [](const sht_tensor* t, int64_t id, const std::vector<int>& axes) {
+    int64_t v = 1;
+    for (int ax : axes) v *= t->bs[ax][(id / t->bstride[ax]) % t->nblk[ax]];
+    return v;
+  };
+  std::vector<int64_t> order(nt);
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<int64_t> aI(A->indices.size()), aK(A->indices.size()), bJ(B->indices.size());
+  for (size_t p = 0; p < A->indices.size(); ++p) {
+    aI[p] = key(A, A->indices[p], m.a_row);
+    aK[p] = key(A, A->indices[p], m.a_col);
+  }
+  for (size_t q = 0; q < B->indices.size(); ++q) bJ[q] = key(B, B->indices[q], m.b_col);
+  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    const int64_t ix = aI[ta[x]], iy = aI[ta[y]];
+    if (ix != iy) return ix < iy;
+    const int64_t kx = aK[ta[x]], ky = aK[ta[y]];
+    if (kx != ky) return kx < ky;
+    return bJ[tb[x]] < bJ[tb[y]];
+  });
+  std::vector<AsPass> passes;
+  std::vector<AsPartner> parts;
+  std::vector<AsUnit> units;
+  std::vector<double> cost;
+  int64_t t = 0;
+  while (t < nt) {
+    const int64_t I = aI[ta[order[t]]];
+    const int64_t first_pass = static_cast<int64_t>(passes.size());
+    const int64_t rows = elems(A, A->indices[ta[order[t]]], m.a_row);
+    double work = 0.0;  // sum over passes of k * 16-column tiles
+    while (t < nt && aI[ta[order[t]]] == I) {
+      const int64_t a = ta[order[t]];
+      AsPass ps{};
+      ps.a_off = A->offsets[a];
+      ps.k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
+      ps.m = static_cast<int32_t>(rows);
+      ps.p0 = static_cast<int32_t>(parts.size());
+      int tiles = 0;
+      while (t < nt && ta[order[t]] == a) {
+        const int64_t b = tb[order[t]];
+        const int n = static_cast<int>(elems(B, B->indices[b], m.b_col));
+        for (int n0 = 0; n0 < n; n0 += 32) {
+          parts.push_back(AsPartner{B->offsets[b], C->offsets[tc[order[t]]], n, n0});
+          tiles += (std::min(32, n - n0) + 15) / 16;
+        }
+        ++t;
+      }
+      ps.np = static_cast<int32_t>(parts.size()) - ps.p0;
+      work += static_cast<double>(ps.k) * tiles;
+      passes.push_back(ps);
+    }
+    const int64_t npass = static_cast<int64_t>(passes.size()) - first_pass;
+    for (int64_t r0 = 0; r0 < rows; r0 += kAsRows) {
+      units.push_back(AsUnit{static_cast<int32_t>(first_pass), static_cast<int32_t>(npass),
+                             static_cast<int32_t>(r0), 0});
+      cost.push_back(work * std::min<int64_t>(kAsRows, rows - r0));
+    }
+  }
+  SHT_REQUIRE(passes.size() < (size_t(1) << 31) && parts.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED,
+              "schedule too large");
+  std::vector<int64_t> uo(units.size());
+  std::iota(uo.begin(), uo.end(), 0);
+  std::stable_sort(uo.begin(), uo.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
+  std::vector<AsUnit> us(units.size());
+  for (size_t i = 0; i < uo.size(); ++i) us[i] = units[uo[i]];
+  Arena& arena = P->ctx->arena;
+  P->d_as_pass = static_cast<AsPass*>(arena.allocate(std::max<size_t>(passes.size(), 1) * sizeof(AsPass)));
+  P->d_as_part = static_cast<AsPartner*>(arena.allocate(std::max<size_t>(parts.size(), 1) * sizeof(AsPartner)));
+  P->d_as_unit = static_cast<AsUnit*>(arena.allocate(std::max<size_t>(us.size(), 1) * sizeof(AsUnit)));
+  if (!passes.empty())
+    SHT_HIP(hipMemcpy(P->d_as_pass, passes.data(), passes.size() * sizeof(AsPass), hipMemcpyHostToDevice));
+  if (!parts.empty())
+    SHT_HIP(hipMemcpy(P->d_as_part, parts.data(), parts.size() * sizeof(AsPartner), hipMemcpyHostToDevice));
+  if (!us.empty()) SHT_HIP(hipMemcpy(P->d_as_unit, us.data(), us.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
+  P->as_nunits = static_cast<int64_t>(us.size());
+  P->as_npass = static_cast<int64_t>(passes.size());
+  P->as_npart = static_cast<int64_t>(parts.size());
+  P->as_built = true;
+}
+
+bool as_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64; }
+
 void setup_staging(sht_plan* P)
 {
   const ContractMaps& m = P->maps;
@@ -608,6 +724,26 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   a.alpha_zero = az ? 1 : 0;
   a.beta_zero = bz ? 1 : 0;
   a.alpha_one = alpha == 1.0 ? 1 : 0;
+  if (P->schedule == 1) {
+    launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
+    if (!az) {
+      AsArgs g{};
+      g.A = Ad;
+      g.B = Bd;
+      g.C = Cd;
+      g.passes = P->d_as_pass;
+      g.partners = P->d_as_part;
+      g.units = P->d_as_unit;
+      g.nunits = P->as_nunits;
+      g.alpha = alpha;
+      g.alpha_one = alpha == 1.0 ? 1 : 0;
+      if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
+      launch_as(AsClass::Mfma64, g, s);
+      if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));
+    }
+    if (P->permC) perm(P->C, P->d_Cp, P->C->d_data, P->orderC, 1);
+    return;
+  }
   for (size_t i = 0; i < P->launches.size(); ++i) {
     const auto& L = P->launches[i];
     a.tiles = P->d_tiles + L.first;
@@ -655,6 +791,11 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
       sht::build_tasks(P.get());
       sht::build_schedule(P.get());
       sht::setup_staging(P.get());
+      const char* e = std::getenv("SHT_SCHED");
+      if (e && std::string(e) == "as" && sht::as_supported(P.get())) {
+        sht::build_as_schedule(P.get());
+        P->schedule = 1;
+      }
     } catch (...) {
       P->release();
       throw;
@@ -678,8 +819,8 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 {
   return guard([&] {
     SHT_REQUIRE(P && nlaunch, SHT_ERR_INVALID, "null argument");
-    const size_t n = P->launches.size();
-    std::vector<hipEvent_t> ev(2 * n);
+    const size_t n = P->schedule == 1 ? 1 : P->launches.size();
+    std::vector<hipEvent_t> ev(2 * std::max<size_t>(n, 1));
     for (auto& e : ev) SHT_HIP(hipEventCreate(&e));
     sht_status st = SHT_OK;
     std::string msg;
@@ -691,7 +832,9 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
         float ms = 0.f;
         SHT_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
         if (ms_out) ms_out[i] = ms;
-        if (kclass_out) kclass_out[i] = static_cast<int>(P->launches[i].k);
+        if (kclass_out)
+          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(sht::AsClass::Mfma64)
+                                           : static_cast<int>(P->launches[i].k);
       }
       *nlaunch = static_cast<int>(n);
     } catch (const Error& e) {
@@ -705,6 +848,7 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 
 const char* sht_kclass_name(int k)
 {
+  if (k == 100) return "as_mfma64";
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
@@ -737,6 +881,69 @@ sht_status sht_plan_graph_replay(sht_plan* P, double alpha, double beta, int rep
       P->graph_beta = beta;
     }
     for (int r = 0; r < replays; ++r) SHT_HIP(hipGraphLaunch(P->graph, s));
+  });
+}
+
+sht_status sht_plan_set_schedule(sht_plan* P, int schedule)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && (schedule == 0 || schedule == 1), SHT_ERR_INVALID, "schedule must be 0 or 1");
+    if (schedule == 1) {
+      SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED, "A-stationary schedule needs SHT_PREC_FP64");
+      sht::build_as_schedule(P);
+    }
+    if (P->schedule != schedule && P->graph) {
+      SHT_HIP(hipGraphExecDestroy(P->graph));
+      SHT_HIP(hipGraphDestroy(P->graph_def));
+      P->graph = nullptr;
+      P->graph_def = nullptr;
+    }
+    P->schedule = schedule;
+  });
+}
+
+sht_status sht_plan_get_schedule(const sht_plan* P, int* schedule)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && schedule, SHT_ERR_INVALID, "null argument");
+    *schedule = P->schedule;
+  });
+}
+
+// Time every applicable schedule (reps executions each, after one warm-up,
+// on `stream`) and keep the fastest.  C is overwritten.
+sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, void* stream, float* ms_out)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && reps > 0, SHT_ERR_INVALID, "bad argument");
+    hipStream_t s = sht::as_stream(stream);
+    std::vector<int> cand{0};
+    if (sht::as_supported(P)) cand.push_back(1);
+    hipEvent_t e0, e1;
+    SHT_HIP(hipEventCreate(&e0));
+    SHT_HIP(hipEventCreate(&e1));
+    int best = P->schedule;
+    float best_ms = 1e30f;
+    for (int c : cand) {
+      if (c == 1) sht::build_as_schedule(P);
+      P->schedule = c;
+      sht::enqueue(P, alpha, beta, s);
+      SHT_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) sht::enqueue(P, alpha, beta, s);
+      SHT_HIP(hipEventRecord(e1, s));
+      SHT_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SHT_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms /= reps;
+      if (ms_out) ms_out[c] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->schedule = best;
   });
 }
 

@@ -55,6 +55,48 @@ int kclass_cols(KClass k);
 
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
 
+// ---- A-stationary row-panel schedule --------------------------------------
+// A pass = one A block (block row I, contracted key K) and its partners: the
+// B blocks (K, J) whose product lands in C block (I, J).  A unit = one
+// 16*RT-row chunk of block row I, walking all passes of I in ascending K, so
+// every A element is read once per unit and C tiles are updated in place in
+// the task order (C id, K) of the task list.
+struct AsPass {
+  int64_t a_off;
+  int32_t k;   // contracted length of the A block
+  int32_t m;   // rows of block row I (= rows of every C block (I, J))
+  int32_t p0;  // first partner
+  int32_t np;  // partner count
+};
+struct AsPartner {
+  int64_t b_off;
+  int64_t c_off;
+  int32_t n;   // columns of the B / C block
+  int32_t n0;  // first column handled by this partner slot (32-column slices)
+};
+struct AsUnit {
+  int32_t pass0;
+  int32_t npass;
+  int32_t r0;
+  int32_t pad;
+};
+struct AsArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  const AsPass* passes;
+  const AsPartner* partners;
+  const AsUnit* units;
+  int64_t nunits;
+  double alpha;
+  int alpha_one;
+};
+enum class AsClass : int { Mfma64 = 0, Count };
+constexpr int kAsRows = 32;  // rows per unit
+void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
+// C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
+void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);
+
 // Block permutation (Utils::reshape semantics per block, Utils.h:315-359):
 // out block = in block with axes reordered by `order`.
 struct PermArgs {

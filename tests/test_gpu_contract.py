@@ -22,9 +22,17 @@ CASES = [
     ("akbl,lcka->bc", "akbl", "lcka", "bc"),
     ("ijk,km->ijm", "ijk", "km", "ijm"),
 ]
-PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64, "fp32": sht.PREC_FP32,
-         "bf16x3": sht.PREC_BF16X3}
-TOL = {"fp64_exact": 0.0, "fp64": 1e-12, "fp32": 1e-6, "bf16x3": 1e-4}
+PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64, "fp64_as": sht.PREC_FP64,
+         "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
+TOL = {"fp64_exact": 0.0, "fp64": 1e-12, "fp64_as": 1e-12, "fp32": 1e-6, "bf16x3": 1e-4}
+
+
+def make_plan(ctx, expr, dA, dB, dC, prec):
+    plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
+    if prec == "fp64_as":
+        plan.set_schedule(1)
+        assert plan.get_schedule() == 1
+    return plan
 
 
 @pytest.fixture(scope="module")
@@ -52,7 +60,7 @@ def test_contract_vs_oracle(ctx, expr, ea, eb, ec, prec):
     t, kk, cids, R = oracle_result(expr, A, B, Cp)
     dA, dB = to_device(ctx, A), to_device(ctx, B)
     dC = sht.Tensor(ctx, Cp.block_sizes)
-    plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
     got = plan.tasks()
     assert np.array_equal(got["a"], t[:, 0]) and np.array_equal(got["b"], t[:, 1])
     assert np.array_equal(got["c"], t[:, 2]) and np.array_equal(got["key"], kk)
@@ -76,11 +84,14 @@ def test_contract_integer_exact_every_form(ctx, expr, ea, eb, ec, prec):
     _, _, _, R = oracle_result(expr, A, B, Cp)
     dA, dB = to_device(ctx, A), to_device(ctx, B)
     dC = sht.Tensor(ctx, Cp.block_sizes)
-    sht.contract(ctx, expr, 1.0, dA, dB, 0.0, dC, PRECS[prec])
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
+    plan.execute(1.0, 0.0)
+    sht.call("sht_device_synchronize")
     assert np.array_equal(dC.download(), R.data)
+    plan.close()
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp64_as"])
 def test_alpha_beta_kept_pattern(ctx, prec):
     rng = np.random.default_rng(7)
     expr = "ijkl,klm->ijm"
@@ -93,7 +104,10 @@ def test_alpha_beta_kept_pattern(ctx, prec):
     _, _, _, R = oracle_result(expr, A, B, Ck, alpha=0.5, beta=-1.25, c0=c0)
     dA, dB = to_device(ctx, A), to_device(ctx, B)
     dC = to_device(ctx, Ck)
-    sht.contract(ctx, expr, 0.5, dA, dB, -1.25, dC, PRECS[prec])
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
+    plan.execute(0.5, -1.25)
+    sht.call("sht_device_synchronize")
+    plan.close()
     assert np.array_equal(dC.get_sparse_info()["indices"], keep)
     out = dC.download()
     if prec == "fp64_exact":
@@ -143,14 +157,14 @@ def test_expression_and_block_size_errors(ctx):
         sht.contract(ctx, "ik,kj->ij", 1.0, A, B3, 0.0, C)
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp64_as", "bf16x3"])
 def test_graph_replay_matches_eager(ctx, prec):
     rng = np.random.default_rng(12)
     expr = "ijk,kl->ijl"
     A, B, Cp = make_case(rng, "ijk", "kl", "ijl", p=0.6, bmin=3, bmax=12)
     dA, dB = to_device(ctx, A), to_device(ctx, B)
     dC = sht.Tensor(ctx, Cp.block_sizes)
-    plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
     plan.execute(1.0, 0.0)
     sht.call("sht_device_synchronize")
     eager = dC.download()
@@ -181,9 +195,9 @@ def test_config0_full_size(ctx):
     dB = sht.Tensor(ctx, bsB)
     dB.reserve_random(seed, 1, 0.3)
     dB.fill_random(seed, 1)
-    for prec in ("fp64_exact", "fp64"):
+    for prec in ("fp64_exact", "fp64", "fp64_as"):
         dC = sht.Tensor(ctx, bs)
-        plan = sht.Plan(ctx, "ijk,kl->ijl", dA, dB, dC, PRECS[prec])
+        plan = make_plan(ctx, "ijk,kl->ijl", dA, dB, dC, prec)
         got = plan.tasks()
         assert len(got["a"]) == len(t)
         assert np.array_equal(got["a"], t[:, 0]) and np.array_equal(got["b"], t[:, 1])
@@ -197,3 +211,19 @@ def test_config0_full_size(ctx):
             assert normwise(out, R.data) <= 1e-12
         plan.close()
         dC.close()
+
+
+def test_autotune_keeps_results(ctx):
+    rng = np.random.default_rng(31)
+    expr = "ijkl,klm->ijm"
+    A, B, Cp = make_case(rng, "ijkl", "klm", "ijm", p=0.5, bmin=3, bmax=12)
+    _, _, _, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    plan = sht.Plan(ctx, expr, dA, dB, dC, sht.PREC_FP64)
+    times = plan.autotune(1.0, 0.0, reps=2)
+    assert set(times) == {0, 1} and plan.get_schedule() in (0, 1)
+    plan.execute(1.0, 0.0)
+    sht.call("sht_device_synchronize")
+    assert normwise(dC.download(), R.data) <= 1e-12
+    plan.close()
