@@ -752,8 +752,9 @@ void orc_gen_values_f64(uint64_t seed, int tid, const orc_bst* T, double lambda,
   TView v(T);
   const uint64_t key = tensor_key(seed, tid);
   double* d = (double*)T->data;
-  std::vector<int> c;
-  for (int64_t s = 0; s < npos; ++s) {
+  const int nth = (int)std::min<int64_t>(16, std::max<int64_t>(1, npos / 64));
+  pool_run(0, npos, nth, [&](int64_t s) {
+    std::vector<int> c;
     const int64_t p = positions ? positions[s] : s;
     const int64_t id = T->indices[p];
     v.coords(id, c);
@@ -768,7 +769,7 @@ void orc_gen_values_f64(uint64_t seed, int tid, const orc_bst* T, double lambda,
       const double u = (double)(h >> 11) * 0x1.0p-53;
       b[x] = (2.0 * u - 1.0) * f;
     }
-  }
+  });
 }
 
 }  // extern "C"

@@ -98,9 +98,10 @@ __global__ __launch_bounds__(64) void k_exact(ExecArgs p)
 }
 
 // ---------------------------------------------------------------------------
-// FP64 MFMA.  Operand layout of v_mfma_f64_16x16x4f64 (wave64):
+// FP64 MFMA.  Operand layout of v_mfma_f64_16x16x4f64 (wave64, measured):
 //   A frag: lane l holds A[l%16][l/16]; B frag: B[l/16][l%16];
-//   D: lane l, item v holds D[4*(l/16)+v][l%16].
+//   D: lane l, item v holds D[4*v + l/16][l%16]  (the f32 form below differs:
+//   D[4*(l/16) + v][l%16]).
 // ---------------------------------------------------------------------------
 template <int CT>
 __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int i = tile.m0 + 16 * rt + 4 * v + lk;
         const int j = tile.n0 + 16 * ct + li;
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * lk + v;
+        const int i = tile.m0 + 16 * rt + 4 * v + lk;
         const int j = tile.n0 + 16 * ct + li;
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
@@ -348,79 +349,85 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
 }
 
 // ---------------------------------------------------------------------------
-// A-stationary FP64 MFMA: one wave per unit (16*RT rows of a block row),
-// partners processed in groups of PG slots of 32 columns; C tiles are read,
-// accumulated over the whole A block and written back (C pre-scaled by beta).
+// A-stationary FP64 MFMA.  Workgroup = 4 waves = one 64-row chunk of a block
+// row; wave w owns rows 16w..16w+15 and walks every pass (A block) of the
+// row in ascending K.  Per pass it sweeps up to kAsSlots 16-column slots at
+// once: the A fragment (16 rows x 4 k) is loaded once per k-step and feeds
+// every slot's MFMA, so each A element leaves HBM once.  The four waves read
+// the same B fragments close together in time (L1 reuse).  C tiles are read,
+// accumulated over the pass and written back (C pre-scaled by beta).
 // ---------------------------------------------------------------------------
-template <int RT, int PG>
-__global__ __launch_bounds__(64) void k_as_mfma64(AsArgs p)
+template <int S>
+__device__ inline void as_sweep(const AsArgs& p, const AsPass& ps, int s_begin, int ns, int r, int li, int lk)
 {
-  const AsUnit u = p.units[blockIdx.x];
-  const int lane = threadIdx.x;
-  const int li = lane & 15, lk = lane >> 4;
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* C = static_cast<double*>(p.C);
+  const double* Ab = A + ps.a_off;
+  const int K = ps.k, M = ps.m;
+  AsSlot sl[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) sl[q] = p.slots[s_begin + (q < ns ? q : 0)];
+  v4d acc[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = r + 4 * v + lk;
+      const int j = sl[q].n0 + li;
+      acc[q][v] = (q < ns && i < M && j < sl[q].n) ? C[sl[q].c_off + i + static_cast<int64_t>(j) * M] : 0.0;
+    }
+  const int ia = r + li;
+  const bool rowok = ia < M;
+  // software pipeline: fragments of step k+4 load while step k multiplies
+  double a_nx = (lk < K && rowok) ? Ab[ia + static_cast<int64_t>(lk) * M] : 0.0;
+  double b_nx[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const int j = sl[q].n0 + li;
+    b_nx[q] = (q < ns && lk < K && j < sl[q].n) ? B[sl[q].b_off + lk + static_cast<int64_t>(j) * K] : 0.0;
+  }
+  for (int k = 0; k < K; k += 4) {
+    double a = a_nx;
+    if (!p.alpha_one) a *= p.alpha;
+    double b[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) b[q] = b_nx[q];
+    const int kn = k + 4 + lk;
+    a_nx = (kn < K && rowok) ? Ab[ia + static_cast<int64_t>(kn) * M] : 0.0;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const int j = sl[q].n0 + li;
+      b_nx[q] = (q < ns && kn < K && j < sl[q].n) ? B[sl[q].b_off + kn + static_cast<int64_t>(j) * K] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+      if (q < ns) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[q], 0, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = r + 4 * v + lk;
+      const int j = sl[q].n0 + li;
+      if (q < ns && i < M && j < sl[q].n) C[sl[q].c_off + i + static_cast<int64_t>(j) * M] = acc[q][v];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_as_mfma64(AsArgs p)
+{
+  const AsUnit u = p.units[blockIdx.x];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r = u.r0 + 16 * wave;
+  if (u.npass == 0) return;
+  if (r >= p.passes[u.pass0].m) return;  // this wave's rows are past the block row
   for (int pi = u.pass0; pi < u.pass0 + u.npass; ++pi) {
     const AsPass ps = p.passes[pi];
-    const double* Ab = A + ps.a_off;
-    const int K = ps.k, M = ps.m;
-    for (int g = 0; g < ps.np; g += PG) {
-      const int ng = min(PG, ps.np - g);
-      AsPartner pt[PG];
-#pragma unroll
-      for (int q = 0; q < PG; ++q) pt[q] = p.partners[ps.p0 + g + (q < ng ? q : 0)];
-      v4d acc[RT][PG][2];
-#pragma unroll
-      for (int q = 0; q < PG; ++q)
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int i = u.r0 + 16 * rt + 4 * lk + v;
-              const int j = pt[q].n0 + 16 * ct + li;
-              acc[rt][q][ct][v] = (q < ng && i < M && j < pt[q].n) ? C[pt[q].c_off + i + static_cast<int64_t>(j) * M]
-                                                                  : 0.0;
-            }
-      for (int k = 0; k < K; k += 4) {
-        const int kk = k + lk;
-        const bool kin = kk < K;
-        double a[RT];
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          const int i = u.r0 + 16 * rt + li;
-          a[rt] = (kin && i < M) ? Ab[i + static_cast<int64_t>(kk) * M] : 0.0;
-          if (!p.alpha_one) a[rt] *= p.alpha;
-        }
-#pragma unroll
-        for (int q = 0; q < PG; ++q) {
-          if (q < ng) {
-            const double* Bb = B + pt[q].b_off;
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-              const int j = pt[q].n0 + 16 * ct + li;
-              const double b = (kin && j < pt[q].n) ? Bb[kk + static_cast<int64_t>(j) * K] : 0.0;
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt)
-                acc[rt][q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b, acc[rt][q][ct], 0, 0, 0);
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < PG; ++q)
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int i = u.r0 + 16 * rt + 4 * lk + v;
-              const int j = pt[q].n0 + 16 * ct + li;
-              if (q < ng && i < M && j < pt[q].n) C[pt[q].c_off + i + static_cast<int64_t>(j) * M] = acc[rt][q][ct][v];
-            }
+    for (int g = 0; g < ps.ns; g += kAsSlots) {
+      const int ns = min(kAsSlots, ps.ns - g);
+      as_sweep<kAsSlots>(p, ps, ps.s0 + g, ns, r, li, lk);
     }
   }
 }
@@ -517,10 +524,10 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
 {
   if (a.nunits <= 0) return;
-  const dim3 g(static_cast<unsigned>(a.nunits)), b(kWave);
-  static_assert(kAsRows == 32, "unit rows are 16 * RT");
+  const dim3 g(static_cast<unsigned>(a.nunits)), b(4 * kWave);
+  static_assert(kAsRows == 64, "unit rows are 4 waves x 16");
   switch (k) {
-    case AsClass::Mfma64: hipLaunchKernelGGL((k_as_mfma64<2, 4>), g, b, 0, s, a); break;
+    case AsClass::Mfma64: hipLaunchKernelGGL(k_as_mfma64, g, b, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }
   SHT_HIP(hipGetLastError());

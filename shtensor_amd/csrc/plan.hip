@@ -286,7 +286,7 @@ struct sht_plan {
   int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary row panels
   bool as_built = false;
   sht::AsPass* d_as_pass = nullptr;
-  sht::AsPartner* d_as_part = nullptr;
+  sht::AsSlot* d_as_part = nullptr;
   sht::AsUnit* d_as_unit = nullptr;
   int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
   hipGraphExec_t graph = nullptr;
@@ -600,7 +600,7 @@ void build_as_schedule(sht_plan* P)
     return bJ[tb[x]] < bJ[tb[y]];
   });
   std::vector<AsPass> passes;
-  std::vector<AsPartner> parts;
+  std::vector<AsSlot> parts;
   std::vector<AsUnit> units;
   std::vector<double> cost;
   int64_t t = 0;
@@ -615,18 +615,18 @@ void build_as_schedule(sht_plan* P)
       ps.a_off = A->offsets[a];
       ps.k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
       ps.m = static_cast<int32_t>(rows);
-      ps.p0 = static_cast<int32_t>(parts.size());
+      ps.s0 = static_cast<int32_t>(parts.size());
       int tiles = 0;
       while (t < nt && ta[order[t]] == a) {
         const int64_t b = tb[order[t]];
         const int n = static_cast<int>(elems(B, B->indices[b], m.b_col));
-        for (int n0 = 0; n0 < n; n0 += 32) {
-          parts.push_back(AsPartner{B->offsets[b], C->offsets[tc[order[t]]], n, n0});
-          tiles += (std::min(32, n - n0) + 15) / 16;
+        for (int n0 = 0; n0 < n; n0 += 16) {
+          parts.push_back(AsSlot{B->offsets[b], C->offsets[tc[order[t]]], n, n0});
+          ++tiles;
         }
         ++t;
       }
-      ps.np = static_cast<int32_t>(parts.size()) - ps.p0;
+      ps.ns = static_cast<int32_t>(parts.size()) - ps.s0;
       work += static_cast<double>(ps.k) * tiles;
       passes.push_back(ps);
     }
@@ -646,12 +646,12 @@ void build_as_schedule(sht_plan* P)
   for (size_t i = 0; i < uo.size(); ++i) us[i] = units[uo[i]];
   Arena& arena = P->ctx->arena;
   P->d_as_pass = static_cast<AsPass*>(arena.allocate(std::max<size_t>(passes.size(), 1) * sizeof(AsPass)));
-  P->d_as_part = static_cast<AsPartner*>(arena.allocate(std::max<size_t>(parts.size(), 1) * sizeof(AsPartner)));
+  P->d_as_part = static_cast<AsSlot*>(arena.allocate(std::max<size_t>(parts.size(), 1) * sizeof(AsSlot)));
   P->d_as_unit = static_cast<AsUnit*>(arena.allocate(std::max<size_t>(us.size(), 1) * sizeof(AsUnit)));
   if (!passes.empty())
     SHT_HIP(hipMemcpy(P->d_as_pass, passes.data(), passes.size() * sizeof(AsPass), hipMemcpyHostToDevice));
   if (!parts.empty())
-    SHT_HIP(hipMemcpy(P->d_as_part, parts.data(), parts.size() * sizeof(AsPartner), hipMemcpyHostToDevice));
+    SHT_HIP(hipMemcpy(P->d_as_part, parts.data(), parts.size() * sizeof(AsSlot), hipMemcpyHostToDevice));
   if (!us.empty()) SHT_HIP(hipMemcpy(P->d_as_unit, us.data(), us.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
   P->as_nunits = static_cast<int64_t>(us.size());
   P->as_npass = static_cast<int64_t>(passes.size());
@@ -732,7 +732,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.B = Bd;
       g.C = Cd;
       g.passes = P->d_as_pass;
-      g.partners = P->d_as_part;
+      g.slots = P->d_as_part;
       g.units = P->d_as_unit;
       g.nunits = P->as_nunits;
       g.alpha = alpha;

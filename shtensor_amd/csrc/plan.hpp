@@ -56,23 +56,23 @@ int kclass_cols(KClass k);
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
 
 // ---- A-stationary row-panel schedule --------------------------------------
-// A pass = one A block (block row I, contracted key K) and its partners: the
-// B blocks (K, J) whose product lands in C block (I, J).  A unit = one
-// 16*RT-row chunk of block row I, walking all passes of I in ascending K, so
-// every A element is read once per unit and C tiles are updated in place in
-// the task order (C id, K) of the task list.
+// A pass = one A block (block row I, contracted key K) with its partners: the
+// B blocks (K, J) whose product lands in C block (I, J), cut into 16-column
+// slots.  A unit = one kAsRows-row chunk of block row I walking all passes of
+// I in ascending K, so every A element is read once per unit and each C tile
+// is updated in place in the task order (C id, K) of the task list.
 struct AsPass {
   int64_t a_off;
   int32_t k;   // contracted length of the A block
   int32_t m;   // rows of block row I (= rows of every C block (I, J))
-  int32_t p0;  // first partner
-  int32_t np;  // partner count
+  int32_t s0;  // first slot
+  int32_t ns;  // slot count
 };
-struct AsPartner {
+struct AsSlot {
   int64_t b_off;
   int64_t c_off;
   int32_t n;   // columns of the B / C block
-  int32_t n0;  // first column handled by this partner slot (32-column slices)
+  int32_t n0;  // first column of this 16-column slot
 };
 struct AsUnit {
   int32_t pass0;
@@ -85,14 +85,15 @@ struct AsArgs {
   const void* B;
   void* C;
   const AsPass* passes;
-  const AsPartner* partners;
+  const AsSlot* slots;
   const AsUnit* units;
   int64_t nunits;
   double alpha;
   int alpha_one;
 };
 enum class AsClass : int { Mfma64 = 0, Count };
-constexpr int kAsRows = 32;  // rows per unit
+constexpr int kAsRows = 64;   // rows per unit: 4 waves x 16 rows
+constexpr int kAsSlots = 8;   // 16-column slots per sweep
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
 void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);
