@@ -23,8 +23,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6e12, "fp64_cs": 78.6e12, "fp64_as": 78.6e12, "fp64_exact": 78.6e12, "fp32": 157.3e12,
-        "bf16x3": 2.5e15}
+PEAK = {"fp64": 78.6e12, "fp64_cs": 78.6e12, "fp64_as": 78.6e12, "fp64_exact": 78.6e12,
+        "fp64_exact_cs": 78.6e12, "fp64_exact_as": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 HBM_PEAK = 8.0e12
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
 
@@ -143,6 +143,9 @@ def main():
     ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time hipGraph replays instead of eager steps")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run C = 1*C (alpha=0) through the A-stationary path: a plain read+write of C "
+                         "(counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -153,7 +156,8 @@ def main():
 
     t0 = time.time()
     precs = {"fp64": sht.PREC_FP64, "fp64_cs": sht.PREC_FP64, "fp64_as": sht.PREC_FP64,
-             "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
+             "fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_cs": sht.PREC_FP64_EXACT,
+             "fp64_exact_as": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
     forms = [f for f in args.forms.split(",") if f]
     arena = int(float(os.environ.get("SHT_BENCH_ARENA_GB", "200")) * (1 << 30))
     ctx = sht.Context(arena)
@@ -169,9 +173,9 @@ def main():
         tp = time.time()
         plan = sht.Plan(ctx, expr, A, B, Cf, precs[f])
         tuned = None
-        if f == "fp64":
+        if f in ("fp64", "fp64_exact"):
             tuned = plan.autotune(1.0, 0.0, reps=2, stream=stream)  # picks the schedule
-        elif f == "fp64_as":
+        elif f in ("fp64_as", "fp64_exact_as"):
             plan.set_schedule(1)
         st = plan.stats()
         st["schedule"] = plan.get_schedule()
@@ -194,6 +198,11 @@ def main():
             stream.synchronize()
             times.append(time.perf_counter() - ts)
         timed = plan.execute_timed(1.0, 0.0, stream)
+        if args.calib and plan.get_schedule() == 1:
+            plan.execute(0.0, 1.0, stream)  # k_scale only: reads and writes every C element once
+            stream.synchronize()
+            log("calibration: C = 1*C over %d elements (%.3e bytes read, same written)" % (Cf.nelem(),
+                                                                                         8.0 * Cf.nelem()))
         times = np.array(times)
         results[f] = {"stats": st, "times": times, "launches": timed, "plan": plan}
         Ctens[f] = Cf

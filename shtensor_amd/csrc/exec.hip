@@ -384,8 +384,8 @@ __device__ inline void as_sweep(const AsArgs& p, const AsPass& ps, int s_begin, 
   double b_nx[S];
 #pragma unroll
   for (int q = 0; q < S; ++q) {
-    const int j = sl[q].n0 + li;
-    b_nx[q] = (q < ns && lk < K && j < sl[q].n) ? B[sl[q].b_off + lk + static_cast<int64_t>(j) * K] : 0.0;
+    const int npad = (sl[q].n + kBtPad - 1) / kBtPad * kBtPad;
+    b_nx[q] = (q < ns && lk < K) ? B[sl[q].bt_off + static_cast<int64_t>(lk) * npad + sl[q].n0 + li] : 0.0;
   }
   for (int k = 0; k < K; k += 4) {
     double a = a_nx;
@@ -397,8 +397,8 @@ __device__ inline void as_sweep(const AsArgs& p, const AsPass& ps, int s_begin, 
     a_nx = (kn < K && rowok) ? Ab[ia + static_cast<int64_t>(kn) * M] : 0.0;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
-      const int j = sl[q].n0 + li;
-      b_nx[q] = (q < ns && kn < K && j < sl[q].n) ? B[sl[q].b_off + kn + static_cast<int64_t>(j) * K] : 0.0;
+      const int npad = (sl[q].n + kBtPad - 1) / kBtPad * kBtPad;
+      b_nx[q] = (q < ns && kn < K) ? B[sl[q].bt_off + static_cast<int64_t>(kn) * npad + sl[q].n0 + li] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < S; ++q)
@@ -428,6 +428,83 @@ __global__ __launch_bounds__(256) void k_as_mfma64(AsArgs p)
     for (int g = 0; g < ps.ns; g += kAsSlots) {
       const int ns = min(kAsSlots, ps.ns - g);
       as_sweep<kAsSlots>(p, ps, ps.s0 + g, ns, r, li, lk);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A-stationary fma chain (bitwise equal to the oracle).  One wave per unit,
+// lane = one C row of the 64-row chunk; per pass, up to kAsxSlots slots of
+// kAsxCols columns keep their C values in registers while k runs ascending:
+// c = fma(alpha*A(i,k), B(k,j), c).  B(k, n0..n0+7) is wave-uniform and
+// contiguous in the transposed block, so it arrives by scalar loads.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_as_exact(AsArgs p)
+{
+  constexpr int S = kAsxSlots, W = kAsxCols;
+  const AsUnit u = p.units[blockIdx.x];
+  const int lane = threadIdx.x;
+  if (u.npass == 0) return;
+  const double* A = static_cast<const double*>(p.A);
+  const double* BT = static_cast<const double*>(p.B);
+  double* C = static_cast<double*>(p.C);
+  const int row = u.r0 + lane;
+  for (int pi = u.pass0; pi < u.pass0 + u.npass; ++pi) {
+    const AsPass ps = p.passes[pi];
+    const int K = ps.k, M = ps.m;
+    const bool rowok = row < M;
+    const double* Ab = A + ps.a_off + row;
+    for (int g = 0; g < ps.ns; g += S) {
+      const int ns = min(S, ps.ns - g);
+      const double* bt[S];
+      int pitch[S];
+      double acc[S][W];
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        const AsSlot sl = p.slots[ps.s0 + g + (q < ns ? q : 0)];
+        pitch[q] = (sl.n + kBtPad - 1) / kBtPad * kBtPad;
+        bt[q] = BT + sl.bt_off + sl.n0;
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          acc[q][j] = (q < ns && rowok && sl.n0 + j < sl.n) ? C[sl.c_off + row + static_cast<int64_t>(sl.n0 + j) * M]
+                                                           : 0.0;
+      }
+      for (int k = 0; k < K; ++k) {
+        double a = rowok ? Ab[static_cast<int64_t>(k) * M] : 0.0;
+        if (!p.alpha_one) a = a * p.alpha;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          if (q < ns) {
+            const double* b = bt[q] + static_cast<int64_t>(k) * pitch[q];
+#pragma unroll
+            for (int j = 0; j < W; ++j) acc[q][j] = __fma_rn(a, b[j], acc[q][j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        if (q < ns && rowok) {
+          const AsSlot sl = p.slots[ps.s0 + g + q];
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            if (sl.n0 + j < sl.n) C[sl.c_off + row + static_cast<int64_t>(sl.n0 + j) * M] = acc[q][j];
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bt(const BtRec* recs, int64_t nrec, const double* B, double* BT)
+{
+  for (int64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
+    const BtRec rec = recs[r];
+    const int npad = (rec.n + kBtPad - 1) / kBtPad * kBtPad;
+    const int64_t tot = static_cast<int64_t>(rec.k) * npad;
+    const double* b = B + rec.b_off;
+    double* o = BT + rec.bt_off;
+    for (int64_t x = threadIdx.x; x < tot; x += blockDim.x) {
+      const int k = static_cast<int>(x / npad), j = static_cast<int>(x - static_cast<int64_t>(k) * npad);
+      o[x] = j < rec.n ? b[k + static_cast<int64_t>(j) * rec.k] : 0.0;
     }
   }
 }
@@ -528,8 +605,17 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
   static_assert(kAsRows == 64, "unit rows are 4 waves x 16");
   switch (k) {
     case AsClass::Mfma64: hipLaunchKernelGGL(k_as_mfma64, g, b, 0, s, a); break;
+    case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g, dim3(kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }
+  SHT_HIP(hipGetLastError());
+}
+
+void launch_bt(const BtRec* recs, int64_t nrec, const double* B, double* BT, hipStream_t s)
+{
+  if (nrec <= 0) return;
+  const int g = static_cast<int>(nrec < 65536 ? nrec : 65536);
+  hipLaunchKernelGGL(k_bt, dim3(g), dim3(256), 0, s, recs, nrec, B, BT);
   SHT_HIP(hipGetLastError());
 }
 

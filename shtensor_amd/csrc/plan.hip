@@ -288,6 +288,9 @@ struct sht_plan {
   sht::AsPass* d_as_pass = nullptr;
   sht::AsSlot* d_as_part = nullptr;
   sht::AsUnit* d_as_unit = nullptr;
+  sht::BtRec* d_bt_rec = nullptr;
+  double* d_bt = nullptr;
+  int64_t bt_nrec = 0;
   int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
@@ -299,7 +302,8 @@ struct sht_plan {
     for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
                     static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
-                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit)})
+                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit), static_cast<void*>(d_bt_rec),
+                    static_cast<void*>(d_bt)})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);
@@ -599,6 +603,18 @@ void build_as_schedule(sht_plan* P)
     if (kx != ky) return kx < ky;
     return bJ[tb[x]] < bJ[tb[y]];
   });
+  // transposed-B layout: one record per B block
+  const int64_t nb = static_cast<int64_t>(B->indices.size());
+  std::vector<BtRec> bt(nb);
+  int64_t bt_total = 0;
+  for (int64_t q = 0; q < nb; ++q) {
+    const int kq = static_cast<int>(elems(B, B->indices[q], m.b_row));
+    const int nq = static_cast<int>(elems(B, B->indices[q], m.b_col));
+    bt[q] = BtRec{B->offsets[q], bt_total, kq, nq};
+    bt_total += static_cast<int64_t>(kq) * ((nq + kBtPad - 1) / kBtPad * kBtPad);
+  }
+  const bool exact = P->prec == SHT_PREC_FP64_EXACT;
+  const int width = exact ? kAsxCols : 16;
   std::vector<AsPass> passes;
   std::vector<AsSlot> parts;
   std::vector<AsUnit> units;
@@ -620,8 +636,8 @@ void build_as_schedule(sht_plan* P)
       while (t < nt && ta[order[t]] == a) {
         const int64_t b = tb[order[t]];
         const int n = static_cast<int>(elems(B, B->indices[b], m.b_col));
-        for (int n0 = 0; n0 < n; n0 += 16) {
-          parts.push_back(AsSlot{B->offsets[b], C->offsets[tc[order[t]]], n, n0});
+        for (int n0 = 0; n0 < n; n0 += width) {
+          parts.push_back(AsSlot{bt[b].bt_off, C->offsets[tc[order[t]]], n, n0});
           ++tiles;
         }
         ++t;
@@ -653,13 +669,17 @@ void build_as_schedule(sht_plan* P)
   if (!parts.empty())
     SHT_HIP(hipMemcpy(P->d_as_part, parts.data(), parts.size() * sizeof(AsSlot), hipMemcpyHostToDevice));
   if (!us.empty()) SHT_HIP(hipMemcpy(P->d_as_unit, us.data(), us.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
+  P->d_bt_rec = static_cast<BtRec*>(arena.allocate(std::max<int64_t>(nb, 1) * sizeof(BtRec)));
+  P->d_bt = static_cast<double*>(arena.allocate(std::max<int64_t>(bt_total, 1) * sizeof(double)));
+  if (nb) SHT_HIP(hipMemcpy(P->d_bt_rec, bt.data(), nb * sizeof(BtRec), hipMemcpyHostToDevice));
+  P->bt_nrec = nb;
   P->as_nunits = static_cast<int64_t>(us.size());
   P->as_npass = static_cast<int64_t>(passes.size());
   P->as_npart = static_cast<int64_t>(parts.size());
   P->as_built = true;
 }
 
-bool as_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64; }
+bool as_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP64_EXACT; }
 
 void setup_staging(sht_plan* P)
 {
@@ -727,9 +747,10 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   if (P->schedule == 1) {
     launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
     if (!az) {
+      launch_bt(P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, s);
       AsArgs g{};
       g.A = Ad;
-      g.B = Bd;
+      g.B = P->d_bt;
       g.C = Cd;
       g.passes = P->d_as_pass;
       g.slots = P->d_as_part;
@@ -738,7 +759,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
-      launch_as(AsClass::Mfma64, g, s);
+      launch_as(P->prec == SHT_PREC_FP64_EXACT ? AsClass::Exact : AsClass::Mfma64, g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));
     }
     if (P->permC) perm(P->C, P->d_Cp, P->C->d_data, P->orderC, 1);
@@ -833,7 +854,9 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
         SHT_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
         if (ms_out) ms_out[i] = ms;
         if (kclass_out)
-          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(sht::AsClass::Mfma64)
+          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(P->prec == SHT_PREC_FP64_EXACT
+                                                                         ? sht::AsClass::Exact
+                                                                         : sht::AsClass::Mfma64)
                                            : static_cast<int>(P->launches[i].k);
       }
       *nlaunch = static_cast<int>(n);
@@ -849,6 +872,7 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 const char* sht_kclass_name(int k)
 {
   if (k == 100) return "as_mfma64";
+  if (k == 101) return "as_exact";
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
@@ -889,7 +913,8 @@ sht_status sht_plan_set_schedule(sht_plan* P, int schedule)
   return guard([&] {
     SHT_REQUIRE(P && (schedule == 0 || schedule == 1), SHT_ERR_INVALID, "schedule must be 0 or 1");
     if (schedule == 1) {
-      SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED, "A-stationary schedule needs SHT_PREC_FP64");
+      SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED,
+                  "A-stationary schedule needs SHT_PREC_FP64 or SHT_PREC_FP64_EXACT");
       sht::build_as_schedule(P);
     }
     if (P->schedule != schedule && P->graph) {

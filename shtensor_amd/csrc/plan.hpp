@@ -69,11 +69,21 @@ struct AsPass {
   int32_t ns;  // slot count
 };
 struct AsSlot {
-  int64_t b_off;
+  int64_t bt_off;  // offset of the partner's transposed B block (k-major, row pitch npad)
   int64_t c_off;
-  int32_t n;   // columns of the B / C block
-  int32_t n0;  // first column of this 16-column slot
+  int32_t n;       // columns of the B / C block
+  int32_t n0;      // first column of this slot
 };
+// B blocks transposed to k-major rows padded to 16 columns (zeros), so a
+// slot's columns for one k are contiguous: coalesced MFMA fragments and
+// wave-uniform scalar loads for the fma-chain kernel.
+constexpr int kBtPad = 16;
+struct BtRec {
+  int64_t b_off;   // canonical B block (k x n, column-major)
+  int64_t bt_off;  // transposed block (k rows of npad)
+  int32_t k, n;
+};
+void launch_bt(const BtRec* recs, int64_t nrec, const double* B, double* BT, hipStream_t s);
 struct AsUnit {
   int32_t pass0;
   int32_t npass;
@@ -91,9 +101,11 @@ struct AsArgs {
   double alpha;
   int alpha_one;
 };
-enum class AsClass : int { Mfma64 = 0, Count };
-constexpr int kAsRows = 64;   // rows per unit: 4 waves x 16 rows
-constexpr int kAsSlots = 8;   // 16-column slots per sweep
+enum class AsClass : int { Mfma64 = 0, Exact = 1, Count };
+constexpr int kAsRows = 64;     // rows per unit (MFMA: 4 waves x 16 rows; exact: 1 wave x 64 rows)
+constexpr int kAsSlots = 8;     // 16-column slots per MFMA sweep
+constexpr int kAsxCols = 8;     // columns per fma-chain slot
+constexpr int kAsxSlots = 8;    // slots per fma-chain sweep
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
 void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);

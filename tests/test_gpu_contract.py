@@ -22,14 +22,15 @@ CASES = [
     ("akbl,lcka->bc", "akbl", "lcka", "bc"),
     ("ijk,km->ijm", "ijk", "km", "ijm"),
 ]
-PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64, "fp64_as": sht.PREC_FP64,
-         "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
-TOL = {"fp64_exact": 0.0, "fp64": 1e-12, "fp64_as": 1e-12, "fp32": 1e-6, "bf16x3": 1e-4}
+PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_as": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64,
+         "fp64_as": sht.PREC_FP64, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
+TOL = {"fp64_exact": 0.0, "fp64_exact_as": 0.0, "fp64": 1e-12, "fp64_as": 1e-12, "fp32": 1e-6, "bf16x3": 1e-4}
+EXACT = ("fp64_exact", "fp64_exact_as")
 
 
 def make_plan(ctx, expr, dA, dB, dC, prec):
     plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
-    if prec == "fp64_as":
+    if prec in ("fp64_as", "fp64_exact_as"):
         plan.set_schedule(1)
         assert plan.get_schedule() == 1
     return plan
@@ -69,7 +70,7 @@ def test_contract_vs_oracle(ctx, expr, ea, eb, ec, prec):
     plan.execute(1.0, 0.0)
     sht.call("sht_device_synchronize")
     out = dC.download()
-    if prec == "fp64_exact":
+    if prec in EXACT:
         assert np.array_equal(out, R.data)
     else:
         assert normwise(out, R.data) <= TOL[prec], normwise(out, R.data)
@@ -91,7 +92,7 @@ def test_contract_integer_exact_every_form(ctx, expr, ea, eb, ec, prec):
     plan.close()
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp64_as"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as", "fp64", "fp64_as"])
 def test_alpha_beta_kept_pattern(ctx, prec):
     rng = np.random.default_rng(7)
     expr = "ijkl,klm->ijm"
@@ -110,7 +111,7 @@ def test_alpha_beta_kept_pattern(ctx, prec):
     plan.close()
     assert np.array_equal(dC.get_sparse_info()["indices"], keep)
     out = dC.download()
-    if prec == "fp64_exact":
+    if prec in EXACT:
         assert np.array_equal(out, R.data)
     else:
         assert normwise(out, R.data) <= 1e-12
@@ -157,7 +158,7 @@ def test_expression_and_block_size_errors(ctx):
         sht.contract(ctx, "ik,kj->ij", 1.0, A, B3, 0.0, C)
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp64_as", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as", "fp64", "fp64_as", "bf16x3"])
 def test_graph_replay_matches_eager(ctx, prec):
     rng = np.random.default_rng(12)
     expr = "ijk,kl->ijl"
@@ -195,7 +196,7 @@ def test_config0_full_size(ctx):
     dB = sht.Tensor(ctx, bsB)
     dB.reserve_random(seed, 1, 0.3)
     dB.fill_random(seed, 1)
-    for prec in ("fp64_exact", "fp64", "fp64_as"):
+    for prec in ("fp64_exact", "fp64_exact_as", "fp64", "fp64_as"):
         dC = sht.Tensor(ctx, bs)
         plan = make_plan(ctx, "ijk,kl->ijl", dA, dB, dC, prec)
         got = plan.tasks()
@@ -205,7 +206,7 @@ def test_config0_full_size(ctx):
         plan.execute()
         sht.call("sht_device_synchronize")
         out = dC.download()
-        if prec == "fp64_exact":
+        if prec in EXACT:
             assert np.array_equal(out, R.data)
         else:
             assert normwise(out, R.data) <= 1e-12
