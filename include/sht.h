@@ -168,7 +168,9 @@ const char* sht_kclass_name(int kclass);
 sht_status sht_plan_set_schedule(sht_plan* p, int schedule);
 sht_status sht_plan_get_schedule(const sht_plan* p, int* schedule);
 /* Autotuner: time each applicable schedule (`reps` executions on `stream`)
- * and keep the fastest; ms_out[schedule] receives each time (may be NULL).
+ * and keep the fastest.  ms_out (3 entries, may be NULL) receives the time
+ * of: [0] C-stationary, [1] A-stationary 8-slot sweeps, [2] A-stationary
+ * 4-slot sweeps (-1 where not applicable).
  * C is overwritten by the trial executions.                                 */
 sht_status sht_plan_autotune(sht_plan* p, double alpha, double beta, int reps, void* stream, float* ms_out);
 /* capture one execution into a hipGraph and replay it `replays` times       */
@@ -180,6 +182,26 @@ sht_status sht_plan_tasks(sht_plan* p, const int64_t** ta, const int64_t** tb, c
 /* useful flops (sum 2*m*n*k) and algorithmic HBM bytes of one execution     */
 sht_status sht_plan_stats(const sht_plan* p, double* flops, double* bytes, int64_t* ntasks, int64_t* ncblocks);
 sht_status sht_plan_destroy(sht_plan* p);
+
+/* ---- serving: continuous batching of contractions ----------------------
+ * Requests name operand tensors resident in the context's arena.  A
+ * dispatcher thread admits requests in arrival order while the summed
+ * footprint (A + B + reserved C bytes) of unfinished requests fits
+ * `budget_bytes`, plans them, and merges the tile lists of all ready
+ * requests with equal (precision, alpha, beta) into shared grouped launches
+ * on its own stream.  A request larger than the budget is rejected with
+ * SHT_ERR_ADMISSION.  C must not be touched until sht_server_wait returns.
+ * Results are bit-identical to a stand-alone sht_contract.                  */
+typedef struct sht_server sht_server;
+sht_status sht_server_create(sht_ctx* ctx, size_t budget_bytes, int max_batch, sht_server** out);
+sht_status sht_server_submit(sht_server* s, const char* expr, double alpha, const sht_tensor* A, const sht_tensor* B,
+                             double beta, sht_tensor* C, sht_precision prec, int64_t* req_id);
+/* blocks until the request finished; latency = completion - submission     */
+sht_status sht_server_wait(sht_server* s, int64_t req_id, double* latency_ms);
+sht_status sht_server_poll(sht_server* s, int64_t req_id, int* done);
+sht_status sht_server_stats(sht_server* s, int64_t* nreq_done, int64_t* nbatches, int64_t* max_merged,
+                            int64_t* nlaunches);
+sht_status sht_server_destroy(sht_server* s);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,7 @@ from . import _lib
 from ._lib import (FLOAT32, FLOAT64, FROBENIUS, LAPACK, MAXABS, PREC_BF16X3, PREC_FP32, PREC_FP64,
                    PREC_FP64_EXACT, XMM, ShtError, call, check, lib)
 
-__all__ = ["Context", "DeviceArray", "Kernel", "Tensor", "Plan", "contract", "contract_info", "ShtError",
+__all__ = ["Context", "DeviceArray", "Kernel", "Tensor", "Plan", "Server", "contract", "contract_info", "ShtError",
            "FLOAT32", "FLOAT64", "LAPACK", "XMM", "FROBENIUS", "MAXABS", "PREC_FP64", "PREC_FP64_EXACT",
            "PREC_FP32", "PREC_BF16X3", "gen_block_sizes", "Stream"]
 
@@ -153,7 +153,7 @@ class Context:
 
     def close(self):
         if self.ptr:
-            for kind in (Plan, Tensor):
+            for kind in (Server, Plan, Tensor):
                 for child in list(self._children):
                     if isinstance(child, kind):
                         child.close()
@@ -362,9 +362,9 @@ class Plan:
 
     def autotune(self, alpha=1.0, beta=0.0, reps=2, stream=None):
         """Time every applicable schedule, keep the fastest; {schedule: ms}."""
-        ms = (C.c_float * 2)(-1.0, -1.0)
+        ms = (C.c_float * 3)(-1.0, -1.0, -1.0)
         call("sht_plan_autotune", C.c_void_p(self.ptr), float(alpha), float(beta), int(reps), _sptr(stream), ms)
-        return {i: float(ms[i]) for i in range(2) if ms[i] >= 0}
+        return {i: float(ms[i]) for i in range(3) if ms[i] >= 0}
 
     def tasks(self):
         ptrs = [C.POINTER(C.c_int64)() for _ in range(4)]
@@ -383,6 +383,50 @@ class Plan:
     def close(self):
         if getattr(self, "ptr", None):
             call("sht_plan_destroy", C.c_void_p(self.ptr))
+            self.ptr = None
+
+    __del__ = close
+
+
+class Server:
+    """Continuous-batching request loop over one context (see sht_server_*)."""
+
+    def __init__(self, ctx, budget_bytes, max_batch=64):
+        p = C.c_void_p()
+        call("sht_server_create", C.c_void_p(ctx.ptr), int(budget_bytes), int(max_batch), C.byref(p))
+        self.ptr = p.value
+        self._keep = {}
+        ctx._children.add(self)
+
+    def submit(self, expr, alpha, A, B, beta, Cten, precision=PREC_FP64):
+        rid = C.c_int64()
+        call("sht_server_submit", C.c_void_p(self.ptr), expr.encode(), float(alpha), C.c_void_p(A.ptr),
+             C.c_void_p(B.ptr), float(beta), C.c_void_p(Cten.ptr), precision, C.byref(rid))
+        self._keep[rid.value] = (A, B, Cten)
+        return rid.value
+
+    def wait(self, rid):
+        """Latency in ms (raises ShtError if the request failed)."""
+        lat = C.c_double()
+        try:
+            call("sht_server_wait", C.c_void_p(self.ptr), int(rid), C.byref(lat))
+        finally:
+            self._keep.pop(rid, None)
+        return lat.value
+
+    def poll(self, rid):
+        d = C.c_int()
+        call("sht_server_poll", C.c_void_p(self.ptr), int(rid), C.byref(d))
+        return bool(d.value)
+
+    def stats(self):
+        v = [C.c_int64() for _ in range(4)]
+        call("sht_server_stats", C.c_void_p(self.ptr), *[C.byref(x) for x in v])
+        return dict(zip(("done", "batches", "max_merged", "launches"), [x.value for x in v]))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            call("sht_server_destroy", C.c_void_p(self.ptr))
             self.ptr = None
 
     __del__ = close

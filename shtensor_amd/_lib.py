@@ -76,6 +76,12 @@ SIGNATURES = {
     "sht_plan_tasks": [_P, C.POINTER(_PI64), C.POINTER(_PI64), C.POINTER(_PI64), C.POINTER(_PI64), _PI64],
     "sht_plan_stats": [_P, C.POINTER(_D), C.POINTER(_D), _PI64, _PI64],
     "sht_plan_destroy": [_P],
+    "sht_server_create": [_P, _SZ, _I, _PP],
+    "sht_server_submit": [_P, C.c_char_p, _D, _P, _P, _D, _P, _I, _PI64],
+    "sht_server_wait": [_P, _I64, C.POINTER(_D)],
+    "sht_server_poll": [_P, _I64, _PI],
+    "sht_server_stats": [_P, _PI64, _PI64, _PI64, _PI64],
+    "sht_server_destroy": [_P],
 }
 
 _lib = None

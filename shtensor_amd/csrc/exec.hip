@@ -360,60 +360,86 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
 template <int S>
 __device__ inline void as_sweep(const AsArgs& p, const AsPass& ps, int s_begin, int ns, int r, int li, int lk)
 {
+  // A fragments are streamed from HBM D k-steps ahead of use (one step's
+  // MFMAs do not cover HBM latency); B fragments come from the transposed,
+  // zero-padded blocks (L2-resident) one step ahead.  Per-slot state is kept
+  // to one B pointer and a pitch; C tile metadata is re-read at the end.
+  constexpr int D = 4;
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* C = static_cast<double*>(p.C);
-  const double* Ab = A + ps.a_off;
   const int K = ps.k, M = ps.m;
-  AsSlot sl[S];
-#pragma unroll
-  for (int q = 0; q < S; ++q) sl[q] = p.slots[s_begin + (q < ns ? q : 0)];
-  v4d acc[S];
-#pragma unroll
-  for (int q = 0; q < S; ++q)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = r + 4 * v + lk;
-      const int j = sl[q].n0 + li;
-      acc[q][v] = (q < ns && i < M && j < sl[q].n) ? C[sl[q].c_off + i + static_cast<int64_t>(j) * M] : 0.0;
-    }
   const int ia = r + li;
   const bool rowok = ia < M;
-  // software pipeline: fragments of step k+4 load while step k multiplies
-  double a_nx = (lk < K && rowok) ? Ab[ia + static_cast<int64_t>(lk) * M] : 0.0;
-  double b_nx[S];
+  const double* Arow = A + ps.a_off + ia;
+  const double* bp[S];
+  int pitch[S];
+  v4d acc[S];
 #pragma unroll
   for (int q = 0; q < S; ++q) {
-    const int npad = (sl[q].n + kBtPad - 1) / kBtPad * kBtPad;
-    b_nx[q] = (q < ns && lk < K) ? B[sl[q].bt_off + static_cast<int64_t>(lk) * npad + sl[q].n0 + li] : 0.0;
-  }
-  for (int k = 0; k < K; k += 4) {
-    double a = a_nx;
-    if (!p.alpha_one) a *= p.alpha;
-    double b[S];
-#pragma unroll
-    for (int q = 0; q < S; ++q) b[q] = b_nx[q];
-    const int kn = k + 4 + lk;
-    a_nx = (kn < K && rowok) ? Ab[ia + static_cast<int64_t>(kn) * M] : 0.0;
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      const int npad = (sl[q].n + kBtPad - 1) / kBtPad * kBtPad;
-      b_nx[q] = (q < ns && kn < K) ? B[sl[q].bt_off + static_cast<int64_t>(kn) * npad + sl[q].n0 + li] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < S; ++q)
-      if (q < ns) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[q], 0, 0, 0);
-  }
-#pragma unroll
-  for (int q = 0; q < S; ++q)
+    const AsSlot sl = p.slots[s_begin + (q < ns ? q : 0)];
+    pitch[q] = (sl.n + kBtPad - 1) / kBtPad * kBtPad;
+    bp[q] = B + sl.bt_off + sl.n0 + li;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int i = r + 4 * v + lk;
-      const int j = sl[q].n0 + li;
-      if (q < ns && i < M && j < sl[q].n) C[sl[q].c_off + i + static_cast<int64_t>(j) * M] = acc[q][v];
+      const int j = sl.n0 + li;
+      acc[q][v] = (q < ns && i < M && j < sl.n) ? C[sl.c_off + i + static_cast<int64_t>(j) * M] : 0.0;
     }
+  }
+  double a_cur[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int kk = 4 * d + lk;
+    a_cur[d] = (kk < K && rowok) ? Arow[static_cast<int64_t>(kk) * M] : 0.0;
+  }
+  double b_nx[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) b_nx[q] = (q < ns && lk < K) ? bp[q][static_cast<int64_t>(lk) * pitch[q]] : 0.0;
+  const int nsteps = (K + 3) >> 2;
+  for (int c = 0; c < nsteps; c += D) {
+    double a_nxt[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int kk = 4 * (c + D + d) + lk;
+      a_nxt[d] = (kk < K && rowok) ? Arow[static_cast<int64_t>(kk) * M] : 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int step = c + d;
+      if (step < nsteps) {
+        double a = a_cur[d];
+        if (!p.alpha_one) a *= p.alpha;
+        double b[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) b[q] = b_nx[q];
+        const int kn = 4 * (step + 1) + lk;
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+          if (q < ns) b_nx[q] = kn < K ? bp[q][static_cast<int64_t>(kn) * pitch[q]] : 0.0;
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+          if (q < ns) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[q], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) a_cur[d] = a_nxt[d];
+  }
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    if (q < ns) {
+      const AsSlot sl = p.slots[s_begin + q];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r + 4 * v + lk;
+        const int j = sl.n0 + li;
+        if (i < M && j < sl.n) C[sl.c_off + i + static_cast<int64_t>(j) * M] = acc[q][v];
+      }
+    }
+  }
 }
 
+template <int S>
 __global__ __launch_bounds__(256) void k_as_mfma64(AsArgs p)
 {
   const AsUnit u = p.units[blockIdx.x];
@@ -425,10 +451,7 @@ __global__ __launch_bounds__(256) void k_as_mfma64(AsArgs p)
   if (r >= p.passes[u.pass0].m) return;  // this wave's rows are past the block row
   for (int pi = u.pass0; pi < u.pass0 + u.npass; ++pi) {
     const AsPass ps = p.passes[pi];
-    for (int g = 0; g < ps.ns; g += kAsSlots) {
-      const int ns = min(kAsSlots, ps.ns - g);
-      as_sweep<kAsSlots>(p, ps, ps.s0 + g, ns, r, li, lk);
-    }
+    for (int g = 0; g < ps.ns; g += S) as_sweep<S>(p, ps, ps.s0 + g, min(S, ps.ns - g), r, li, lk);
   }
 }
 
@@ -604,7 +627,8 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
   const dim3 g(static_cast<unsigned>(a.nunits)), b(4 * kWave);
   static_assert(kAsRows == 64, "unit rows are 4 waves x 16");
   switch (k) {
-    case AsClass::Mfma64: hipLaunchKernelGGL(k_as_mfma64, g, b, 0, s, a); break;
+    case AsClass::Mfma64: hipLaunchKernelGGL(k_as_mfma64<kAsSlots>, g, b, 0, s, a); break;
+    case AsClass::Mfma64s4: hipLaunchKernelGGL(k_as_mfma64<4>, g, b, 0, s, a); break;
     case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g, dim3(kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }

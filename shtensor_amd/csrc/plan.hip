@@ -248,68 +248,6 @@ AxisList axis_list(const std::vector<int>& v)
 
 }  // namespace sht
 
-struct sht_plan {
-  sht_ctx* ctx = nullptr;
-  std::string expr;
-  const sht_tensor* A = nullptr;
-  const sht_tensor* B = nullptr;
-  sht_tensor* C = nullptr;
-  sht_precision prec = SHT_PREC_FP64;
-  sht::ContractMaps maps;
-  int64_t ntasks = 0, ncb = 0;
-  // device (arena)
-  int64_t* d_ta = nullptr;
-  int64_t* d_tb = nullptr;
-  int64_t* d_tc = nullptr;
-  int64_t* d_kk = nullptr;
-  sht::TaskRec* d_tasks = nullptr;
-  sht::CBlkRec* d_cblk = nullptr;
-  sht::TileRec* d_tiles = nullptr;
-  struct Launch {
-    sht::KClass k;
-    int64_t first, count;
-  };
-  std::vector<Launch> launches;
-  // staging for non-canonical layouts
-  bool permA = false, permB = false, permC = false;
-  void* d_Ap = nullptr;
-  void* d_Bp = nullptr;
-  void* d_Cp = nullptr;
-  int orderA[sht::kMaxModes], orderB[sht::kMaxModes], orderC[sht::kMaxModes];
-  // host copies
-  std::vector<int64_t> h_ta, h_tb, h_tc, h_kk;
-  bool host_tasks = false;
-  double flops = 0.0;
-  double bytes_in = 0.0;  // A + B + C write
-  double bytes_c = 0.0;   // C read (beta != 0)
-  // A-stationary schedule (built on demand)
-  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary row panels
-  bool as_built = false;
-  sht::AsPass* d_as_pass = nullptr;
-  sht::AsSlot* d_as_part = nullptr;
-  sht::AsUnit* d_as_unit = nullptr;
-  sht::BtRec* d_bt_rec = nullptr;
-  double* d_bt = nullptr;
-  int64_t bt_nrec = 0;
-  int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
-  hipGraphExec_t graph = nullptr;
-  hipGraph_t graph_def = nullptr;
-  double graph_alpha = 0.0, graph_beta = 0.0;
-
-  void release()
-  {
-    sht::Arena& a = ctx->arena;
-    for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
-                    static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
-                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
-                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit), static_cast<void*>(d_bt_rec),
-                    static_cast<void*>(d_bt)})
-      if (p) a.release(p);
-    if (graph) (void)hipGraphExecDestroy(graph);
-    if (graph_def) (void)hipGraphDestroy(graph_def);
-  }
-};
-
 namespace sht {
 
 namespace {
@@ -701,49 +639,49 @@ void setup_staging(sht_plan* P)
   if (P->permC) P->d_Cp = arena.allocate(std::max<int64_t>(P->C->nelem, 1) * 8);
 }
 
-void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<hipEvent_t>* ev = nullptr)
+}  // namespace
+
+namespace {
+
+bool zero_bytes(double v)
 {
-  auto zero_bytes = [](double v) {
-    uint64_t u;
-    std::memcpy(&u, &v, 8);
-    return u == 0;
-  };
+  uint64_t u;
+  std::memcpy(&u, &v, 8);
+  return u == 0;
+}
+
+void permute(const sht_tensor* t, const void* in, void* out, const int* order, int inverse, hipStream_t s)
+{
+  PermArgs pa{};
+  pa.d = t->desc();
+  pa.in = in;
+  pa.out = out;
+  for (int i = 0; i < t->N; ++i) pa.order[i] = order[i];
+  pa.inverse = inverse;
+  launch_permute(pa, true, s);
+}
+
+}  // namespace
+
+void enqueue_stage_in(sht_plan* P, double alpha, double beta, hipStream_t s)
+{
+  if (P->permA && !zero_bytes(alpha)) permute(P->A, P->A->d_data, P->d_Ap, P->orderA, 0, s);
+  if (P->permB && !zero_bytes(alpha)) permute(P->B, P->B->d_data, P->d_Bp, P->orderB, 0, s);
+  if (P->permC && !zero_bytes(beta)) permute(P->C, P->C->d_data, P->d_Cp, P->orderC, 0, s);
+}
+
+void enqueue_stage_out(sht_plan* P, hipStream_t s)
+{
+  if (P->permC) permute(P->C, P->d_Cp, P->C->d_data, P->orderC, 1, s);
+}
+
+void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<hipEvent_t>* ev)
+{
   const bool az = zero_bytes(alpha), bz = zero_bytes(beta);
-  const void* Ad = P->A->d_data;
-  const void* Bd = P->B->d_data;
-  void* Cd = P->C->d_data;
-  auto perm = [&](const sht_tensor* t, const void* in, void* out, const int* order, int inverse) {
-    PermArgs pa{};
-    pa.d = t->desc();
-    pa.in = in;
-    pa.out = out;
-    for (int i = 0; i < t->N; ++i) pa.order[i] = order[i];
-    pa.inverse = inverse;
-    launch_permute(pa, true, s);
-  };
-  if (P->permA && !az) {
-    perm(P->A, Ad, P->d_Ap, P->orderA, 0);
-    Ad = P->d_Ap;
-  }
-  if (P->permB && !az) {
-    perm(P->B, Bd, P->d_Bp, P->orderB, 0);
-    Bd = P->d_Bp;
-  }
-  if (P->permC) {
-    if (!bz) perm(P->C, Cd, P->d_Cp, P->orderC, 0);
-    Cd = P->d_Cp;
-  }
-  ExecArgs a{};
-  a.A = Ad;
-  a.B = Bd;
-  a.C = Cd;
-  a.cblk = P->d_cblk;
-  a.tasks = P->d_tasks;
-  a.alpha = alpha;
-  a.beta = beta;
-  a.alpha_zero = az ? 1 : 0;
-  a.beta_zero = bz ? 1 : 0;
-  a.alpha_one = alpha == 1.0 ? 1 : 0;
+  enqueue_stage_in(P, alpha, beta, s);
+  const void* Ad = P->permA ? P->d_Ap : P->A->d_data;
+  const void* Bd = P->permB ? P->d_Bp : P->B->d_data;
+  void* Cd = P->permC ? P->d_Cp : P->C->d_data;
   if (P->schedule == 1) {
     launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
     if (!az) {
@@ -759,12 +697,25 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
-      launch_as(P->prec == SHT_PREC_FP64_EXACT ? AsClass::Exact : AsClass::Mfma64, g, s);
+      launch_as(P->prec == SHT_PREC_FP64_EXACT ? AsClass::Exact
+                                               : (P->as_variant ? AsClass::Mfma64s4 : AsClass::Mfma64),
+                g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));
     }
-    if (P->permC) perm(P->C, P->d_Cp, P->C->d_data, P->orderC, 1);
+    enqueue_stage_out(P, s);
     return;
   }
+  ExecArgs a{};
+  a.A = Ad;
+  a.B = Bd;
+  a.C = Cd;
+  a.cblk = P->d_cblk;
+  a.tasks = P->d_tasks;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.alpha_zero = az ? 1 : 0;
+  a.beta_zero = bz ? 1 : 0;
+  a.alpha_one = alpha == 1.0 ? 1 : 0;
   for (size_t i = 0; i < P->launches.size(); ++i) {
     const auto& L = P->launches[i];
     a.tiles = P->d_tiles + L.first;
@@ -773,10 +724,8 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
     launch_exec(L.k, a, s);
     if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i + 1], s));
   }
-  if (P->permC) perm(P->C, P->d_Cp, P->C->d_data, P->orderC, 1);
+  enqueue_stage_out(P, s);
 }
-
-}  // namespace
 
 }  // namespace sht
 
@@ -854,9 +803,9 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
         SHT_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
         if (ms_out) ms_out[i] = ms;
         if (kclass_out)
-          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(P->prec == SHT_PREC_FP64_EXACT
-                                                                         ? sht::AsClass::Exact
-                                                                         : sht::AsClass::Mfma64)
+          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(P->prec == SHT_PREC_FP64_EXACT ? sht::AsClass::Exact
+                                                                    : P->as_variant ? sht::AsClass::Mfma64s4
+                                                                                    : sht::AsClass::Mfma64)
                                            : static_cast<int>(P->launches[i].k);
       }
       *nlaunch = static_cast<int>(n);
@@ -873,6 +822,7 @@ const char* sht_kclass_name(int k)
 {
   if (k == 100) return "as_mfma64";
   if (k == 101) return "as_exact";
+  if (k == 102) return "as_mfma64_s4";
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
@@ -944,14 +894,17 @@ sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, v
     hipStream_t s = sht::as_stream(stream);
     std::vector<int> cand{0};
     if (sht::as_supported(P)) cand.push_back(1);
+    if (P->prec == SHT_PREC_FP64) cand.push_back(2);  // A-stationary, 4-slot sweeps
     hipEvent_t e0, e1;
     SHT_HIP(hipEventCreate(&e0));
     SHT_HIP(hipEventCreate(&e1));
     int best = P->schedule;
     float best_ms = 1e30f;
+    int best_variant = P->as_variant;
     for (int c : cand) {
-      if (c == 1) sht::build_as_schedule(P);
-      P->schedule = c;
+      if (c >= 1) sht::build_as_schedule(P);
+      P->schedule = c >= 1 ? 1 : 0;
+      P->as_variant = c == 2 ? 1 : 0;
       sht::enqueue(P, alpha, beta, s);
       SHT_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < reps; ++r) sht::enqueue(P, alpha, beta, s);
@@ -963,12 +916,14 @@ sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, v
       if (ms_out) ms_out[c] = ms;
       if (ms < best_ms) {
         best_ms = ms;
-        best = c;
+        best = c >= 1 ? 1 : 0;
+        best_variant = c == 2 ? 1 : 0;
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->schedule = best;
+    P->as_variant = best_variant;
   });
 }
 

@@ -1,6 +1,9 @@
 // Contraction plan: task list, C pattern and execution schedule on the GPU.
 #pragma once
 
+#include <string>
+#include <vector>
+
 #include "internal.hpp"
 
 namespace sht {
@@ -101,7 +104,7 @@ struct AsArgs {
   double alpha;
   int alpha_one;
 };
-enum class AsClass : int { Mfma64 = 0, Exact = 1, Count };
+enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64s4 = 2, Count };
 constexpr int kAsRows = 64;     // rows per unit (MFMA: 4 waves x 16 rows; exact: 1 wave x 64 rows)
 constexpr int kAsSlots = 8;     // 16-column slots per MFMA sweep
 constexpr int kAsxCols = 8;     // columns per fma-chain slot
@@ -121,4 +124,75 @@ struct PermArgs {
 };
 void launch_permute(const PermArgs& p, bool f64, hipStream_t s);
 
+}  // namespace sht
+
+struct sht_plan {
+  sht_ctx* ctx = nullptr;
+  std::string expr;
+  const sht_tensor* A = nullptr;
+  const sht_tensor* B = nullptr;
+  sht_tensor* C = nullptr;
+  sht_precision prec = SHT_PREC_FP64;
+  sht::ContractMaps maps;
+  int64_t ntasks = 0, ncb = 0;
+  // device (arena)
+  int64_t* d_ta = nullptr;
+  int64_t* d_tb = nullptr;
+  int64_t* d_tc = nullptr;
+  int64_t* d_kk = nullptr;
+  sht::TaskRec* d_tasks = nullptr;
+  sht::CBlkRec* d_cblk = nullptr;
+  sht::TileRec* d_tiles = nullptr;
+  struct Launch {
+    sht::KClass k;
+    int64_t first, count;
+  };
+  std::vector<Launch> launches;
+  // staging for non-canonical layouts
+  bool permA = false, permB = false, permC = false;
+  void* d_Ap = nullptr;
+  void* d_Bp = nullptr;
+  void* d_Cp = nullptr;
+  int orderA[sht::kMaxModes], orderB[sht::kMaxModes], orderC[sht::kMaxModes];
+  // host copies
+  std::vector<int64_t> h_ta, h_tb, h_tc, h_kk;
+  bool host_tasks = false;
+  double flops = 0.0;
+  double bytes_in = 0.0;  // A + B + C write
+  double bytes_c = 0.0;   // C read (beta != 0)
+  // A-stationary schedule (built on demand)
+  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary row panels
+  int as_variant = 0;  // A-stationary MFMA kernel: 0 = 8 slots per sweep, 1 = 4 slots
+  bool as_built = false;
+  sht::AsPass* d_as_pass = nullptr;
+  sht::AsSlot* d_as_part = nullptr;
+  sht::AsUnit* d_as_unit = nullptr;
+  sht::BtRec* d_bt_rec = nullptr;
+  double* d_bt = nullptr;
+  int64_t bt_nrec = 0;
+  int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  double graph_alpha = 0.0, graph_beta = 0.0;
+
+  void release()
+  {
+    sht::Arena& a = ctx->arena;
+    for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
+                    static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
+                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
+                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit), static_cast<void*>(d_bt_rec),
+                    static_cast<void*>(d_bt)})
+      if (p) a.release(p);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_def) (void)hipGraphDestroy(graph_def);
+  }
+};
+
+namespace sht {
+// Enqueue one execution of P on s (events around grouped launches when ev).
+void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<hipEvent_t>* ev = nullptr);
+// The permutation staging parts of an execution (non-canonical layouts).
+void enqueue_stage_in(sht_plan* P, double alpha, double beta, hipStream_t s);
+void enqueue_stage_out(sht_plan* P, hipStream_t s);
 }  // namespace sht

@@ -120,7 +120,7 @@ void tensor_reserve_sorted(sht_tensor* t, std::vector<int64_t>&& ids)
     SHT_HIP(hipMemcpy(t->d_offsets, t->offsets.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
   }
   launch_zero(t->d_data, static_cast<size_t>(std::max<int64_t>(off, 1)) * t->elem_size(), nullptr);
-  SHT_HIP(hipDeviceSynchronize());
+  SHT_HIP(hipStreamSynchronize(nullptr));
   t->reserved = true;
 }
 

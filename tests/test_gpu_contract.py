@@ -24,8 +24,11 @@ CASES = [
 ]
 PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_as": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64,
          "fp64_as": sht.PREC_FP64, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
-TOL = {"fp64_exact": 0.0, "fp64_exact_as": 0.0, "fp64": 1e-12, "fp64_as": 1e-12, "fp32": 1e-6, "bf16x3": 1e-4}
-EXACT = ("fp64_exact", "fp64_exact_as")
+TOL = {"fp64_exact": 0.0, "fp64_exact_as": 0.0, "fp64": 0.0, "fp64_as": 0.0, "fp32": 1e-6, "bf16x3": 1e-4}
+# v_mfma_f64_16x16x4f64 accumulates its four products as a sequential fused
+# multiply-add chain in k order, so the FP64 MFMA forms equal the fma chain
+# bit for bit (asserted here; the stated contract of SHT_PREC_FP64 is 1e-12).
+EXACT = ("fp64_exact", "fp64_exact_as", "fp64", "fp64_as")
 
 
 def make_plan(ctx, expr, dA, dB, dC, prec):
@@ -223,7 +226,7 @@ def test_autotune_keeps_results(ctx):
     dC = sht.Tensor(ctx, Cp.block_sizes)
     plan = sht.Plan(ctx, expr, dA, dB, dC, sht.PREC_FP64)
     times = plan.autotune(1.0, 0.0, reps=2)
-    assert set(times) == {0, 1} and plan.get_schedule() in (0, 1)
+    assert set(times) == {0, 1, 2} and plan.get_schedule() in (0, 1)
     plan.execute(1.0, 0.0)
     sht.call("sht_device_synchronize")
     assert normwise(dC.download(), R.data) <= 1e-12
