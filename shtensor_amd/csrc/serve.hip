@@ -338,10 +338,11 @@ sht_status sht_server_submit(sht_server* S, const char* expr, double alpha, cons
                                          " bytes exceeds the admission budget of " + std::to_string(S->budget));
     r->t_submit = sht::now_ms();
     std::lock_guard<std::mutex> lk(S->mu);
-    r->id = S->next_id++;
-    *req_id = r->id;
-    S->pending.push_back(r.get());
-    S->reqs[r->id] = r.release();
+    sht::Request* raw = r.release();
+    raw->id = S->next_id++;
+    *req_id = raw->id;
+    S->reqs[raw->id] = raw;
+    S->pending.push_back(raw);
     S->cv_work.notify_all();
   });
 }
