@@ -24,7 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp64_cs": 78.6e12, "fp64_as": 78.6e12, "fp64_exact": 78.6e12,
-        "fp64_exact_cs": 78.6e12, "fp64_exact_as": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+        "fp64_exact_cs": 78.6e12, "fp64_exact_as": 78.6e12, "fp32": 157.3e12, "fp32_cs": 157.3e12,
+        "fp32_as": 157.3e12, "bf16x3": 2.5e15, "bf16x3_cs": 2.5e15, "bf16x3_as": 2.5e15}
 HBM_PEAK = 8.0e12
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
 
@@ -137,7 +138,7 @@ def main():
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--length", type=int, default=None, help="override every mode length (smaller runs)")
     ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3",
-                    help="fp64 (autotuned schedule), fp64_cs, fp64_as, fp64_exact, fp32, bf16x3")
+                    help="fp64 (autotuned schedule), fp64_cs, fp64_as, fp64_as1, fp64_exact, fp32, bf16x3; _cs = C-stationary, _as = A-stationary 4-wave, _as1 = A-stationary one-wave units")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
@@ -155,9 +156,11 @@ def main():
     import shtensor_amd as sht
 
     t0 = time.time()
-    precs = {"fp64": sht.PREC_FP64, "fp64_cs": sht.PREC_FP64, "fp64_as": sht.PREC_FP64,
+    precs = {"fp64": sht.PREC_FP64, "fp64_cs": sht.PREC_FP64, "fp64_as": sht.PREC_FP64, "fp64_as1": sht.PREC_FP64,
              "fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_cs": sht.PREC_FP64_EXACT,
-             "fp64_exact_as": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
+             "fp64_exact_as": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "fp32_cs": sht.PREC_FP32,
+             "fp32_as": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3, "bf16x3_cs": sht.PREC_BF16X3,
+             "bf16x3_as": sht.PREC_BF16X3}
     forms = [f for f in args.forms.split(",") if f]
     arena = int(float(os.environ.get("SHT_BENCH_ARENA_GB", "200")) * (1 << 30))
     ctx = sht.Context(arena)
@@ -173,10 +176,12 @@ def main():
         tp = time.time()
         plan = sht.Plan(ctx, expr, A, B, Cf, precs[f])
         tuned = None
-        if f in ("fp64", "fp64_exact"):
+        if f in ("fp64", "fp64_exact", "fp32", "bf16x3"):
             tuned = plan.autotune(1.0, 0.0, reps=2, stream=stream)  # picks the schedule
-        elif f in ("fp64_as", "fp64_exact_as"):
+        elif f.endswith("_as"):
             plan.set_schedule(1)
+        elif f.endswith("_as1"):
+            plan.set_schedule(2)
         st = plan.stats()
         st["schedule"] = plan.get_schedule()
         st["autotune_ms"] = tuned

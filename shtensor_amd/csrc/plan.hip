@@ -545,30 +545,35 @@ void build_as_schedule(sht_plan* P)
   const int64_t nb = static_cast<int64_t>(B->indices.size());
   std::vector<BtRec> bt(nb);
   int64_t bt_total = 0;
+  const bool bf3 = P->prec == SHT_PREC_BF16X3;
   for (int64_t q = 0; q < nb; ++q) {
     const int kq = static_cast<int>(elems(B, B->indices[q], m.b_row));
     const int nq = static_cast<int>(elems(B, B->indices[q], m.b_col));
     bt[q] = BtRec{B->offsets[q], bt_total, kq, nq};
-    bt_total += static_cast<int64_t>(kq) * ((nq + kBtPad - 1) / kBtPad * kBtPad);
+    const int64_t krows = bf3 ? (kq + kBfK - 1) / kBfK * kBfK : (kq + 3) / 4 * 4;
+    bt_total += krows * ((nq + kBtPad - 1) / kBtPad * kBtPad);
   }
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
   const int width = exact ? kAsxCols : 16;
   std::vector<AsPass> passes;
   std::vector<AsSlot> parts;
-  std::vector<AsUnit> units;
-  std::vector<double> cost;
+  // block rows: their passes and cost; units are built per block row below
+  struct Row {
+    int32_t pass0, npass;
+    int64_t rows;
+    double work;  // sum over passes of k * slots
+  };
+  std::vector<Row> brows;
   int64_t t = 0;
   while (t < nt) {
     const int64_t I = aI[ta[order[t]]];
-    const int64_t first_pass = static_cast<int64_t>(passes.size());
-    const int64_t rows = elems(A, A->indices[ta[order[t]]], m.a_row);
-    double work = 0.0;  // sum over passes of k * 16-column tiles
+    Row row{static_cast<int32_t>(passes.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
     while (t < nt && aI[ta[order[t]]] == I) {
       const int64_t a = ta[order[t]];
       AsPass ps{};
       ps.a_off = A->offsets[a];
       ps.k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
-      ps.m = static_cast<int32_t>(rows);
+      ps.m = static_cast<int32_t>(row.rows);
       ps.s0 = static_cast<int32_t>(parts.size());
       int tiles = 0;
       while (t < nt && ta[order[t]] == a) {
@@ -581,34 +586,73 @@ void build_as_schedule(sht_plan* P)
         ++t;
       }
       ps.ns = static_cast<int32_t>(parts.size()) - ps.s0;
-      work += static_cast<double>(ps.k) * tiles;
+      row.work += static_cast<double>(ps.k) * tiles;
       passes.push_back(ps);
     }
-    const int64_t npass = static_cast<int64_t>(passes.size()) - first_pass;
-    for (int64_t r0 = 0; r0 < rows; r0 += kAsRows) {
-      units.push_back(AsUnit{static_cast<int32_t>(first_pass), static_cast<int32_t>(npass),
-                             static_cast<int32_t>(r0), 0});
-      cost.push_back(work * std::min<int64_t>(kAsRows, rows - r0));
-    }
+    row.npass = static_cast<int32_t>(passes.size()) - row.pass0;
+    brows.push_back(row);
   }
   SHT_REQUIRE(passes.size() < (size_t(1) << 31) && parts.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED,
               "schedule too large");
-  std::vector<int64_t> uo(units.size());
-  std::iota(uo.begin(), uo.end(), 0);
-  std::stable_sort(uo.begin(), uo.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
-  std::vector<AsUnit> us(units.size());
-  for (size_t i = 0; i < uo.size(); ++i) us[i] = units[uo[i]];
+  // XCD-aware order.  Workgroup g runs on XCD g % 8 (round-robin dispatch),
+  // and every XCD has its own L2: all units of a block row go to one XCD, so
+  // the row's B partners are shared in that L2.  Rows are dealt to the XCD
+  // with the least accumulated work (longest first), then the eight queues
+  // are interleaved; short queues are padded with empty units.
+  constexpr int kXcd = 8;
+  auto build = [&](int step, int per_wg) {
+    std::vector<int64_t> ro(brows.size());
+    std::iota(ro.begin(), ro.end(), 0);
+    std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) {
+      return brows[x].work * brows[x].rows > brows[y].work * brows[y].rows;
+    });
+    std::vector<std::vector<AsUnit>> q(kXcd);
+    std::vector<double> load(kXcd, 0.0);
+    for (int64_t r : ro) {
+      const Row& br = brows[r];
+      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      int in_wg = 0;
+      for (int64_t r0 = 0; r0 < br.rows; r0 += step) {
+        q[x].push_back(AsUnit{br.pass0, br.npass, static_cast<int32_t>(r0), 0});
+        in_wg = (in_wg + 1) % per_wg;
+      }
+      while (in_wg != 0) {  // a workgroup never mixes block rows' XCD queues
+        q[x].push_back(AsUnit{0, 0, 0, 0});
+        in_wg = (in_wg + 1) % per_wg;
+      }
+      load[x] += br.work * static_cast<double>(br.rows);
+    }
+    size_t longest = 0;
+    for (const auto& v : q) longest = std::max(longest, v.size());
+    std::vector<AsUnit> out;
+    out.reserve(longest * kXcd);
+    for (size_t slot = 0; slot < longest; slot += per_wg)
+      for (int x = 0; x < kXcd; ++x)
+        for (int w = 0; w < per_wg; ++w)
+          out.push_back(slot + w < q[x].size() ? q[x][slot + w] : AsUnit{0, 0, 0, 0});
+    while (!out.empty() && out.back().npass == 0) out.pop_back();
+    return out;
+  };
+  const bool exact_rows = P->prec == SHT_PREC_FP64_EXACT;
+  std::vector<AsUnit> us = build(exact_rows ? 64 : kAsRows, exact_rows ? 1 : 4);
+  std::vector<AsUnit> us1 = exact_rows ? std::vector<AsUnit>() : build(kAsRows, 1);
   Arena& arena = P->ctx->arena;
   P->d_as_pass = static_cast<AsPass*>(arena.allocate(std::max<size_t>(passes.size(), 1) * sizeof(AsPass)));
   P->d_as_part = static_cast<AsSlot*>(arena.allocate(std::max<size_t>(parts.size(), 1) * sizeof(AsSlot)));
   P->d_as_unit = static_cast<AsUnit*>(arena.allocate(std::max<size_t>(us.size(), 1) * sizeof(AsUnit)));
+  P->d_as_unit1 = static_cast<AsUnit*>(arena.allocate(std::max<size_t>(us1.size(), 1) * sizeof(AsUnit)));
   if (!passes.empty())
     SHT_HIP(hipMemcpy(P->d_as_pass, passes.data(), passes.size() * sizeof(AsPass), hipMemcpyHostToDevice));
   if (!parts.empty())
     SHT_HIP(hipMemcpy(P->d_as_part, parts.data(), parts.size() * sizeof(AsSlot), hipMemcpyHostToDevice));
   if (!us.empty()) SHT_HIP(hipMemcpy(P->d_as_unit, us.data(), us.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
+  if (!us1.empty())
+    SHT_HIP(hipMemcpy(P->d_as_unit1, us1.data(), us1.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
+  P->as_nunits1 = static_cast<int64_t>(us1.size());
   P->d_bt_rec = static_cast<BtRec*>(arena.allocate(std::max<int64_t>(nb, 1) * sizeof(BtRec)));
-  P->d_bt = static_cast<double*>(arena.allocate(std::max<int64_t>(bt_total, 1) * sizeof(double)));
+  const size_t bt_bytes = bf3 ? 3 * sizeof(uint16_t) : (P->prec == SHT_PREC_FP32 ? sizeof(float) : sizeof(double));
+  P->d_bt = arena.allocate(std::max<int64_t>(bt_total, 1) * bt_bytes);
+  P->bt_total = bt_total;
   if (nb) SHT_HIP(hipMemcpy(P->d_bt_rec, bt.data(), nb * sizeof(BtRec), hipMemcpyHostToDevice));
   P->bt_nrec = nb;
   P->as_nunits = static_cast<int64_t>(us.size());
@@ -617,7 +661,22 @@ void build_as_schedule(sht_plan* P)
   P->as_built = true;
 }
 
-bool as_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP64_EXACT; }
+bool as_supported(const sht_plan*) { return true; }
+
+BtFormat bt_format(const sht_plan* P)
+{
+  return P->prec == SHT_PREC_FP32 ? BtFormat::F32T : (P->prec == SHT_PREC_BF16X3 ? BtFormat::BF16X3 : BtFormat::F64T);
+}
+
+AsClass as_class(const sht_plan* P)
+{
+  switch (P->prec) {
+    case SHT_PREC_FP64_EXACT: return AsClass::Exact;
+    case SHT_PREC_FP32: return AsClass::Mfma32;
+    case SHT_PREC_BF16X3: return AsClass::Bf16x3;
+    default: return P->as_variant ? AsClass::Mfma64s4 : AsClass::Mfma64;
+  }
+}
 
 void setup_staging(sht_plan* P)
 {
@@ -685,21 +744,21 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   if (P->schedule == 1) {
     launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
     if (!az) {
-      launch_bt(P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, s);
+      launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, P->bt_total, s);
       AsArgs g{};
       g.A = Ad;
       g.B = P->d_bt;
+      g.bt_total = P->bt_total;
       g.C = Cd;
       g.passes = P->d_as_pass;
       g.slots = P->d_as_part;
-      g.units = P->d_as_unit;
-      g.nunits = P->as_nunits;
+      const bool one_wave = as_class(P) == AsClass::Mfma64s4;
+      g.units = one_wave ? P->d_as_unit1 : P->d_as_unit;
+      g.nunits = one_wave ? P->as_nunits1 : P->as_nunits;
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
-      launch_as(P->prec == SHT_PREC_FP64_EXACT ? AsClass::Exact
-                                               : (P->as_variant ? AsClass::Mfma64s4 : AsClass::Mfma64),
-                g, s);
+      launch_as(as_class(P), g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));
     }
     enqueue_stage_out(P, s);
@@ -803,9 +862,7 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
         SHT_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
         if (ms_out) ms_out[i] = ms;
         if (kclass_out)
-          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(P->prec == SHT_PREC_FP64_EXACT ? sht::AsClass::Exact
-                                                                    : P->as_variant ? sht::AsClass::Mfma64s4
-                                                                                    : sht::AsClass::Mfma64)
+          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(sht::as_class(P))
                                            : static_cast<int>(P->launches[i].k);
       }
       *nlaunch = static_cast<int>(n);
@@ -823,6 +880,8 @@ const char* sht_kclass_name(int k)
   if (k == 100) return "as_mfma64";
   if (k == 101) return "as_exact";
   if (k == 102) return "as_mfma64_s4";
+  if (k == 103) return "as_mfma32";
+  if (k == 104) return "as_bf16x3";
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
@@ -863,8 +922,7 @@ sht_status sht_plan_set_schedule(sht_plan* P, int schedule)
   return guard([&] {
     SHT_REQUIRE(P && (schedule == 0 || schedule == 1), SHT_ERR_INVALID, "schedule must be 0 or 1");
     if (schedule == 1) {
-      SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED,
-                  "A-stationary schedule needs SHT_PREC_FP64 or SHT_PREC_FP64_EXACT");
+      SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED, "A-stationary schedule not available");
       sht::build_as_schedule(P);
     }
     if (P->schedule != schedule && P->graph) {

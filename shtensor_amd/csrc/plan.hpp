@@ -81,12 +81,22 @@ struct AsSlot {
 // slot's columns for one k are contiguous: coalesced MFMA fragments and
 // wave-uniform scalar loads for the fma-chain kernel.
 constexpr int kBtPad = 16;
+constexpr int kBfK = 32;  // k granularity of the bf16 MFMA (16x16x32)
 struct BtRec {
   int64_t b_off;   // canonical B block (k x n, column-major)
-  int64_t bt_off;  // transposed block (k rows of npad)
+  int64_t bt_off;  // converted block, in elements of the format
   int32_t k, n;
 };
-void launch_bt(const BtRec* recs, int64_t nrec, const double* B, double* BT, hipStream_t s);
+// Per-execution B layouts of the A-stationary kernels:
+//  F64T / F32T: k-major rows of npad = roundup(n, 16) columns, roundup(k, 4)
+//               rows (zero padded);
+//  BF16X3: three bf16 planes (hi, mid, lo) of x = hi + mid + lo, each stored
+//          column-major with the contracted dim padded to 32 (zeros), so one
+//          16-byte load gives the 8 consecutive k of one column an MFMA lane
+//          needs.  Planes are bt_total elements apart.
+enum class BtFormat : int { F64T = 0, F32T = 1, BF16X3 = 2 };
+void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const double* B, void* BT, int64_t bt_total,
+               hipStream_t s);
 struct AsUnit {
   int32_t pass0;
   int32_t npass;
@@ -95,8 +105,9 @@ struct AsUnit {
 };
 struct AsArgs {
   const void* A;
-  const void* B;
+  const void* B;   // converted B (BtFormat of the plan)
   void* C;
+  int64_t bt_total;  // elements per bf16 plane
   const AsPass* passes;
   const AsSlot* slots;
   const AsUnit* units;
@@ -104,8 +115,8 @@ struct AsArgs {
   double alpha;
   int alpha_one;
 };
-enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64s4 = 2, Count };
-constexpr int kAsRows = 64;     // rows per unit (MFMA: 4 waves x 16 rows; exact: 1 wave x 64 rows)
+enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64s4 = 2, Mfma32 = 3, Bf16x3 = 4, Count };
+constexpr int kAsRows = 16;     // rows per MFMA unit (one wave); the fma-chain kernel uses 64-row units
 constexpr int kAsSlots = 8;     // 16-column slots per MFMA sweep
 constexpr int kAsxCols = 8;     // columns per fma-chain slot
 constexpr int kAsxSlots = 8;    // slots per fma-chain sweep
@@ -162,14 +173,16 @@ struct sht_plan {
   double bytes_c = 0.0;   // C read (beta != 0)
   // A-stationary schedule (built on demand)
   int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary row panels
-  int as_variant = 0;  // A-stationary MFMA kernel: 0 = 8 slots per sweep, 1 = 4 slots
+  int as_variant = 0;  // A-stationary FP64 MFMA kernel: 0 = 4-wave groups, 1 = one-wave lean units
   bool as_built = false;
   sht::AsPass* d_as_pass = nullptr;
   sht::AsSlot* d_as_part = nullptr;
-  sht::AsUnit* d_as_unit = nullptr;
+  sht::AsUnit* d_as_unit = nullptr;   // 4-wave workgroups (64-row units for the fma chain)
+  sht::AsUnit* d_as_unit1 = nullptr;  // one-wave workgroups
+  int64_t as_nunits1 = 0;
   sht::BtRec* d_bt_rec = nullptr;
-  double* d_bt = nullptr;
-  int64_t bt_nrec = 0;
+  void* d_bt = nullptr;
+  int64_t bt_nrec = 0, bt_total = 0;
   int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
@@ -181,8 +194,8 @@ struct sht_plan {
     for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
                     static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
-                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit), static_cast<void*>(d_bt_rec),
-                    static_cast<void*>(d_bt)})
+                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit), static_cast<void*>(d_as_unit1),
+                    static_cast<void*>(d_bt_rec), d_bt})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);

@@ -23,8 +23,10 @@ CASES = [
     ("ijk,km->ijm", "ijk", "km", "ijm"),
 ]
 PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_as": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64,
-         "fp64_as": sht.PREC_FP64, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
-TOL = {"fp64_exact": 0.0, "fp64_exact_as": 0.0, "fp64": 0.0, "fp64_as": 0.0, "fp32": 1e-6, "bf16x3": 1e-4}
+         "fp64_as": sht.PREC_FP64, "fp32": sht.PREC_FP32, "fp32_as": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3,
+         "bf16x3_as": sht.PREC_BF16X3}
+TOL = {"fp64_exact": 0.0, "fp64_exact_as": 0.0, "fp64": 0.0, "fp64_as": 0.0, "fp32": 1e-6, "fp32_as": 1e-6,
+       "bf16x3": 1e-6, "bf16x3_as": 1e-6}
 # v_mfma_f64_16x16x4f64 accumulates its four products as a sequential fused
 # multiply-add chain in k order, so the FP64 MFMA forms equal the fma chain
 # bit for bit (asserted here; the stated contract of SHT_PREC_FP64 is 1e-12).
@@ -33,7 +35,7 @@ EXACT = ("fp64_exact", "fp64_exact_as", "fp64", "fp64_as")
 
 def make_plan(ctx, expr, dA, dB, dC, prec):
     plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
-    if prec in ("fp64_as", "fp64_exact_as"):
+    if prec.endswith("_as"):
         plan.set_schedule(1)
         assert plan.get_schedule() == 1
     return plan
@@ -161,7 +163,8 @@ def test_expression_and_block_size_errors(ctx):
         sht.contract(ctx, "ik,kj->ij", 1.0, A, B3, 0.0, C)
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as", "fp64", "fp64_as", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as", "fp64", "fp64_as", "fp32_as", "bf16x3",
+                                  "bf16x3_as"])
 def test_graph_replay_matches_eager(ctx, prec):
     rng = np.random.default_rng(12)
     expr = "ijk,kl->ijl"
