@@ -164,13 +164,14 @@ sht_status sht_plan_execute_timed(sht_plan* p, double alpha, double beta, void* 
 const char* sht_kclass_name(int kclass);
 /* Execution schedule: 0 = C-stationary tiles (one tile per workgroup, C
  * block accumulated in registers over its tasks), 1 = A-stationary row
- * panels (each A element read once per row chunk; SHT_PREC_FP64 only).     */
+ * panels in 4-wave workgroups (each A element read once per 16-row chunk),
+ * 2 = A-stationary row panels in one-wave workgroups (for SHT_PREC_FP64_EXACT
+ * the same kernel as 1).                                                   */
 sht_status sht_plan_set_schedule(sht_plan* p, int schedule);
 sht_status sht_plan_get_schedule(const sht_plan* p, int* schedule);
 /* Autotuner: time each applicable schedule (`reps` executions on `stream`)
  * and keep the fastest.  ms_out (3 entries, may be NULL) receives the time
- * of: [0] C-stationary, [1] A-stationary 8-slot sweeps, [2] A-stationary
- * 4-slot sweeps (-1 where not applicable).
+ * of schedules 0, 1 and 2 (-1 where not tried).
  * C is overwritten by the trial executions.                                 */
 sht_status sht_plan_autotune(sht_plan* p, double alpha, double beta, int reps, void* stream, float* ms_out);
 /* capture one execution into a hipGraph and replay it `replays` times       */

@@ -15,6 +15,10 @@
 namespace sht {
 
 constexpr int kMaxModes = 8;
+// Tensor data buffers are allocated with this many bytes of slack after the
+// last element: the row-chunk kernels may read (never write) up to 64 rows
+// past the end of a block column.
+constexpr size_t kReadSlack = 1024;
 
 // Block-level view of a tensor handed to kernels by value.
 struct TensorDesc {

@@ -497,8 +497,11 @@ void build_schedule(sht_plan* P)
 }
 
 // A-stationary schedule from the task list: tasks regrouped by (block row I,
-// contracted key K, block column J); one pass per A block, one unit per
-// kAsRows-row chunk of each block row, units ordered by descending cost.
+// contracted key K, block column J).  One panel per contracted key holding
+// the key's partners (B blocks that occur in a task) side by side; one pass
+// per A block, with the C block offset of each panel partner (-1 when the
+// product is not kept); units are row chunks of each block row, ordered
+// XCD-aware.
 void build_as_schedule(sht_plan* P)
 {
   if (P->as_built) return;
@@ -528,12 +531,16 @@ void build_as_schedule(sht_plan* P)
   };
   std::vector<int64_t> order(nt);
   std::iota(order.begin(), order.end(), 0);
-  std::vector<int64_t> aI(A->indices.size()), aK(A->indices.size()), bJ(B->indices.size());
-  for (size_t p = 0; p < A->indices.size(); ++p) {
+  const size_t na = A->indices.size(), nbk = B->indices.size();
+  std::vector<int64_t> aI(na), aK(na), bK(nbk), bJ(nbk);
+  for (size_t p = 0; p < na; ++p) {
     aI[p] = key(A, A->indices[p], m.a_row);
     aK[p] = key(A, A->indices[p], m.a_col);
   }
-  for (size_t q = 0; q < B->indices.size(); ++q) bJ[q] = key(B, B->indices[q], m.b_col);
+  for (size_t q = 0; q < nbk; ++q) {
+    bK[q] = key(B, B->indices[q], m.b_row);  // same rolled key as aK (contracted axes in A's order)
+    bJ[q] = key(B, B->indices[q], m.b_col);
+  }
   std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
     const int64_t ix = aI[ta[x]], iy = aI[ta[y]];
     if (ix != iy) return ix < iy;
@@ -541,27 +548,81 @@ void build_as_schedule(sht_plan* P)
     if (kx != ky) return kx < ky;
     return bJ[tb[x]] < bJ[tb[y]];
   });
-  // transposed-B layout: one record per B block
-  const int64_t nb = static_cast<int64_t>(B->indices.size());
-  std::vector<BtRec> bt(nb);
-  int64_t bt_total = 0;
   const bool bf3 = P->prec == SHT_PREC_BF16X3;
-  for (int64_t q = 0; q < nb; ++q) {
-    const int kq = static_cast<int>(elems(B, B->indices[q], m.b_row));
-    const int nq = static_cast<int>(elems(B, B->indices[q], m.b_col));
-    bt[q] = BtRec{B->offsets[q], bt_total, kq, nq};
-    const int64_t krows = bf3 ? (kq + kBfK - 1) / kBfK * kBfK : (kq + 3) / 4 * 4;
-    bt_total += krows * ((nq + kBtPad - 1) / kBtPad * kBtPad);
-  }
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
-  const int width = exact ? kAsxCols : 16;
+  // panels: B blocks that occur in a task, grouped by key, ascending J
+  std::vector<int64_t> used;
+  {
+    std::vector<char> seen(nbk, 0);
+    for (int64_t t = 0; t < nt; ++t)
+      if (!seen[tb[t]]) {
+        seen[tb[t]] = 1;
+        used.push_back(tb[t]);
+      }
+  }
+  std::sort(used.begin(), used.end(), [&](int64_t x, int64_t y) {
+    if (bK[x] != bK[y]) return bK[x] < bK[y];
+    return bJ[x] < bJ[y];
+  });
+  struct Panel {
+    int64_t bt_off;
+    int32_t k, ncol, pitch, col0, npart;
+  };
+  std::vector<Panel> panels;
+  std::vector<int32_t> panel_of(nbk, -1), part_of(nbk, -1);
+  std::vector<BtRec> bt;
+  std::vector<AsCol> cols;
+  int64_t bt_total = 0;
+  for (size_t u = 0; u < used.size();) {
+    size_t e = u;
+    while (e < used.size() && bK[used[e]] == bK[used[u]]) ++e;
+    Panel pn{};
+    pn.k = static_cast<int32_t>(elems(B, B->indices[used[u]], m.b_row));
+    pn.col0 = static_cast<int32_t>(cols.size());
+    int64_t ncol = 0;
+    for (size_t x = u; x < e; ++x) {
+      const int64_t q = used[x];
+      const int32_t n = static_cast<int32_t>(elems(B, B->indices[q], m.b_col));
+      panel_of[q] = static_cast<int32_t>(panels.size());
+      part_of[q] = static_cast<int32_t>(x - u);
+      for (int32_t j = 0; j < n; ++j) cols.push_back(AsCol{static_cast<int32_t>(x - u), j});
+      ncol += n;
+    }
+    SHT_REQUIRE(ncol < (int64_t(1) << 30), SHT_ERR_UNSUPPORTED, "panel too wide");
+    pn.ncol = static_cast<int32_t>(ncol);
+    pn.npart = static_cast<int32_t>(e - u);
+    const int64_t npad = (ncol + kBtPad - 1) / kBtPad * kBtPad;
+    pn.bt_off = bt_total;
+    int64_t c = 0;
+    if (bf3) {
+      pn.pitch = (pn.k + kBfK - 1) / kBfK * kBfK;
+      for (size_t x = u; x < e; ++x) {
+        const int64_t q = used[x];
+        const int32_t n = static_cast<int32_t>(elems(B, B->indices[q], m.b_col));
+        bt.push_back(BtRec{B->offsets[q], pn.bt_off + c * pn.pitch, pn.k, n, pn.pitch, 0});
+        c += n;
+      }
+      bt_total += npad * pn.pitch;
+    } else {
+      pn.pitch = static_cast<int32_t>(npad);
+      for (size_t x = u; x < e; ++x) {
+        const int64_t q = used[x];
+        const int32_t n = static_cast<int32_t>(elems(B, B->indices[q], m.b_col));
+        bt.push_back(BtRec{B->offsets[q], pn.bt_off + c, pn.k, n, pn.pitch, 0});
+        c += n;
+      }
+      bt_total += static_cast<int64_t>((pn.k + 3) / 4 * 4) * npad;
+    }
+    panels.push_back(pn);
+    u = e;
+  }
+  // passes and per-pass partner C offsets; block rows with their cost
   std::vector<AsPass> passes;
-  std::vector<AsSlot> parts;
-  // block rows: their passes and cost; units are built per block row below
+  std::vector<int64_t> cofs;
   struct Row {
     int32_t pass0, npass;
     int64_t rows;
-    double work;  // sum over passes of k * slots
+    double work;  // sum over passes of k * panel columns
   };
   std::vector<Row> brows;
   int64_t t = 0;
@@ -570,37 +631,37 @@ void build_as_schedule(sht_plan* P)
     Row row{static_cast<int32_t>(passes.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
     while (t < nt && aI[ta[order[t]]] == I) {
       const int64_t a = ta[order[t]];
+      const Panel& pn = panels[panel_of[tb[order[t]]]];
       AsPass ps{};
       ps.a_off = A->offsets[a];
+      ps.bt_off = pn.bt_off;
       ps.k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
       ps.m = static_cast<int32_t>(row.rows);
-      ps.s0 = static_cast<int32_t>(parts.size());
-      int tiles = 0;
+      ps.ncol = pn.ncol;
+      ps.pitch = pn.pitch;
+      ps.col0 = pn.col0;
+      ps.cof0 = static_cast<int32_t>(cofs.size());
+      SHT_REQUIRE(cofs.size() + pn.npart < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
+      cofs.resize(cofs.size() + pn.npart, -1);
       while (t < nt && ta[order[t]] == a) {
-        const int64_t b = tb[order[t]];
-        const int n = static_cast<int>(elems(B, B->indices[b], m.b_col));
-        for (int n0 = 0; n0 < n; n0 += width) {
-          parts.push_back(AsSlot{bt[b].bt_off, C->offsets[tc[order[t]]], n, n0});
-          ++tiles;
-        }
+        cofs[ps.cof0 + part_of[tb[order[t]]]] = C->offsets[tc[order[t]]];
         ++t;
       }
-      ps.ns = static_cast<int32_t>(parts.size()) - ps.s0;
-      row.work += static_cast<double>(ps.k) * tiles;
+      row.work += static_cast<double>(ps.k) * ((pn.ncol + kBtPad - 1) / kBtPad * kBtPad);
       passes.push_back(ps);
     }
     row.npass = static_cast<int32_t>(passes.size()) - row.pass0;
     brows.push_back(row);
   }
-  SHT_REQUIRE(passes.size() < (size_t(1) << 31) && parts.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED,
+  SHT_REQUIRE(passes.size() < (size_t(1) << 31) && cols.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED,
               "schedule too large");
   // XCD-aware order.  Workgroup g runs on XCD g % 8 (round-robin dispatch),
   // and every XCD has its own L2: all units of a block row go to one XCD, so
-  // the row's B partners are shared in that L2.  Rows are dealt to the XCD
-  // with the least accumulated work (longest first), then the eight queues
-  // are interleaved; short queues are padded with empty units.
+  // the row's panels are shared in that L2.  Rows are dealt to the XCD with
+  // the least accumulated work (longest first), then the eight queues are
+  // interleaved; short queues are padded with empty units.
   constexpr int kXcd = 8;
-  auto build = [&](int step, int per_wg) {
+  auto build = [&](int step) {
     std::vector<int64_t> ro(brows.size());
     std::iota(ro.begin(), ro.end(), 0);
     std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) {
@@ -611,53 +672,45 @@ void build_as_schedule(sht_plan* P)
     for (int64_t r : ro) {
       const Row& br = brows[r];
       const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-      int in_wg = 0;
-      for (int64_t r0 = 0; r0 < br.rows; r0 += step) {
+      for (int64_t r0 = 0; r0 < br.rows; r0 += step)
         q[x].push_back(AsUnit{br.pass0, br.npass, static_cast<int32_t>(r0), 0});
-        in_wg = (in_wg + 1) % per_wg;
-      }
-      while (in_wg != 0) {  // a workgroup never mixes block rows' XCD queues
-        q[x].push_back(AsUnit{0, 0, 0, 0});
-        in_wg = (in_wg + 1) % per_wg;
-      }
       load[x] += br.work * static_cast<double>(br.rows);
     }
     size_t longest = 0;
     for (const auto& v : q) longest = std::max(longest, v.size());
     std::vector<AsUnit> out;
     out.reserve(longest * kXcd);
-    for (size_t slot = 0; slot < longest; slot += per_wg)
-      for (int x = 0; x < kXcd; ++x)
-        for (int w = 0; w < per_wg; ++w)
-          out.push_back(slot + w < q[x].size() ? q[x][slot + w] : AsUnit{0, 0, 0, 0});
+    for (size_t slot = 0; slot < longest; ++slot)
+      for (int x = 0; x < kXcd; ++x) out.push_back(slot < q[x].size() ? q[x][slot] : AsUnit{0, 0, 0, 0});
     while (!out.empty() && out.back().npass == 0) out.pop_back();
     return out;
   };
-  const bool exact_rows = P->prec == SHT_PREC_FP64_EXACT;
-  std::vector<AsUnit> us = build(exact_rows ? 64 : kAsRows, exact_rows ? 1 : 4);
-  std::vector<AsUnit> us1 = exact_rows ? std::vector<AsUnit>() : build(kAsRows, 1);
+  std::vector<AsUnit> us = exact ? std::vector<AsUnit>() : build(kAsUnitRows);
+  std::vector<AsUnit> usw = build(kAsUnitRowsW);
   Arena& arena = P->ctx->arena;
-  P->d_as_pass = static_cast<AsPass*>(arena.allocate(std::max<size_t>(passes.size(), 1) * sizeof(AsPass)));
-  P->d_as_part = static_cast<AsSlot*>(arena.allocate(std::max<size_t>(parts.size(), 1) * sizeof(AsSlot)));
-  P->d_as_unit = static_cast<AsUnit*>(arena.allocate(std::max<size_t>(us.size(), 1) * sizeof(AsUnit)));
-  P->d_as_unit1 = static_cast<AsUnit*>(arena.allocate(std::max<size_t>(us1.size(), 1) * sizeof(AsUnit)));
-  if (!passes.empty())
-    SHT_HIP(hipMemcpy(P->d_as_pass, passes.data(), passes.size() * sizeof(AsPass), hipMemcpyHostToDevice));
-  if (!parts.empty())
-    SHT_HIP(hipMemcpy(P->d_as_part, parts.data(), parts.size() * sizeof(AsSlot), hipMemcpyHostToDevice));
-  if (!us.empty()) SHT_HIP(hipMemcpy(P->d_as_unit, us.data(), us.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
-  if (!us1.empty())
-    SHT_HIP(hipMemcpy(P->d_as_unit1, us1.data(), us1.size() * sizeof(AsUnit), hipMemcpyHostToDevice));
-  P->as_nunits1 = static_cast<int64_t>(us1.size());
-  P->d_bt_rec = static_cast<BtRec*>(arena.allocate(std::max<int64_t>(nb, 1) * sizeof(BtRec)));
+  auto upload = [&](const void* src, size_t bytes) {
+    void* d = arena.allocate(std::max<size_t>(bytes, 1));
+    if (bytes) SHT_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    return d;
+  };
+  P->d_as_pass = static_cast<AsPass*>(upload(passes.data(), passes.size() * sizeof(AsPass)));
+  P->d_as_col = static_cast<AsCol*>(upload(cols.data(), cols.size() * sizeof(AsCol)));
+  P->d_as_cof = static_cast<int64_t*>(upload(cofs.data(), cofs.size() * sizeof(int64_t)));
+  P->d_as_unit = static_cast<AsUnit*>(upload(us.data(), us.size() * sizeof(AsUnit)));
+  P->d_as_unitw = static_cast<AsUnit*>(upload(usw.data(), usw.size() * sizeof(AsUnit)));
+  P->d_bt_rec = static_cast<BtRec*>(upload(bt.data(), bt.size() * sizeof(BtRec)));
   const size_t bt_bytes = bf3 ? 3 * sizeof(uint16_t) : (P->prec == SHT_PREC_FP32 ? sizeof(float) : sizeof(double));
   P->d_bt = arena.allocate(std::max<int64_t>(bt_total, 1) * bt_bytes);
+  // pad columns and rows of the panels stay zero: the conversion writes only
+  // the blocks' own columns
+  SHT_HIP(hipMemset(P->d_bt, 0, std::max<int64_t>(bt_total, 1) * bt_bytes));
+  SHT_HIP(hipDeviceSynchronize());
   P->bt_total = bt_total;
-  if (nb) SHT_HIP(hipMemcpy(P->d_bt_rec, bt.data(), nb * sizeof(BtRec), hipMemcpyHostToDevice));
-  P->bt_nrec = nb;
+  P->bt_nrec = static_cast<int64_t>(bt.size());
   P->as_nunits = static_cast<int64_t>(us.size());
+  P->as_nunitsw = static_cast<int64_t>(usw.size());
   P->as_npass = static_cast<int64_t>(passes.size());
-  P->as_npart = static_cast<int64_t>(parts.size());
+  P->as_npanel = static_cast<int64_t>(panels.size());
   P->as_built = true;
 }
 
@@ -670,11 +723,12 @@ BtFormat bt_format(const sht_plan* P)
 
 AsClass as_class(const sht_plan* P)
 {
+  const bool wide = P->schedule == 2;
   switch (P->prec) {
     case SHT_PREC_FP64_EXACT: return AsClass::Exact;
-    case SHT_PREC_FP32: return AsClass::Mfma32;
-    case SHT_PREC_BF16X3: return AsClass::Bf16x3;
-    default: return P->as_variant ? AsClass::Mfma64s4 : AsClass::Mfma64;
+    case SHT_PREC_FP32: return wide ? AsClass::Mfma32W : AsClass::Mfma32;
+    case SHT_PREC_BF16X3: return wide ? AsClass::Bf16x3W : AsClass::Bf16x3;
+    default: return wide ? AsClass::Mfma64W : AsClass::Mfma64;
   }
 }
 
@@ -693,9 +747,9 @@ void setup_staging(sht_plan* P)
   order(m.b_row, m.b_col, P->orderB);
   order(m.c_row, m.c_col, P->orderC);
   Arena& arena = P->ctx->arena;
-  if (P->permA) P->d_Ap = arena.allocate(std::max<int64_t>(P->A->nelem, 1) * 8);
-  if (P->permB) P->d_Bp = arena.allocate(std::max<int64_t>(P->B->nelem, 1) * 8);
-  if (P->permC) P->d_Cp = arena.allocate(std::max<int64_t>(P->C->nelem, 1) * 8);
+  if (P->permA) P->d_Ap = arena.allocate(std::max<int64_t>(P->A->nelem, 1) * 8 + kReadSlack);
+  if (P->permB) P->d_Bp = arena.allocate(std::max<int64_t>(P->B->nelem, 1) * 8 + kReadSlack);
+  if (P->permC) P->d_Cp = arena.allocate(std::max<int64_t>(P->C->nelem, 1) * 8 + kReadSlack);
 }
 
 }  // namespace
@@ -741,7 +795,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   const void* Ad = P->permA ? P->d_Ap : P->A->d_data;
   const void* Bd = P->permB ? P->d_Bp : P->B->d_data;
   void* Cd = P->permC ? P->d_Cp : P->C->d_data;
-  if (P->schedule == 1) {
+  if (P->schedule >= 1) {
     launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
     if (!az) {
       launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, P->bt_total, s);
@@ -751,14 +805,16 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.bt_total = P->bt_total;
       g.C = Cd;
       g.passes = P->d_as_pass;
-      g.slots = P->d_as_part;
-      const bool one_wave = as_class(P) == AsClass::Mfma64s4;
-      g.units = one_wave ? P->d_as_unit1 : P->d_as_unit;
-      g.nunits = one_wave ? P->as_nunits1 : P->as_nunits;
+      g.cols = P->d_as_col;
+      g.cofs = P->d_as_cof;
+      const AsClass cls = as_class(P);
+      const bool wide = cls == AsClass::Exact || P->schedule == 2;
+      g.units = wide ? P->d_as_unitw : P->d_as_unit;
+      g.nunits = wide ? P->as_nunitsw : P->as_nunits;
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
-      launch_as(as_class(P), g, s);
+      launch_as(cls, g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));
     }
     enqueue_stage_out(P, s);
@@ -848,7 +904,7 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 {
   return guard([&] {
     SHT_REQUIRE(P && nlaunch, SHT_ERR_INVALID, "null argument");
-    const size_t n = P->schedule == 1 ? 1 : P->launches.size();
+    const size_t n = P->schedule >= 1 ? 1 : P->launches.size();
     std::vector<hipEvent_t> ev(2 * std::max<size_t>(n, 1));
     for (auto& e : ev) SHT_HIP(hipEventCreate(&e));
     sht_status st = SHT_OK;
@@ -862,7 +918,7 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
         SHT_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
         if (ms_out) ms_out[i] = ms;
         if (kclass_out)
-          kclass_out[i] = P->schedule == 1 ? 100 + static_cast<int>(sht::as_class(P))
+          kclass_out[i] = P->schedule >= 1 ? 100 + static_cast<int>(sht::as_class(P))
                                            : static_cast<int>(P->launches[i].k);
       }
       *nlaunch = static_cast<int>(n);
@@ -877,11 +933,9 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 
 const char* sht_kclass_name(int k)
 {
-  if (k == 100) return "as_mfma64";
-  if (k == 101) return "as_exact";
-  if (k == 102) return "as_mfma64_s4";
-  if (k == 103) return "as_mfma32";
-  if (k == 104) return "as_bf16x3";
+  static const char* as_names[] = {"as_mfma64", "as_exact", "as_mfma64_w", "as_mfma32",
+                                   "as_bf16x3", "as_mfma32_w", "as_bf16x3_w"};
+  if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
@@ -920,8 +974,8 @@ sht_status sht_plan_graph_replay(sht_plan* P, double alpha, double beta, int rep
 sht_status sht_plan_set_schedule(sht_plan* P, int schedule)
 {
   return guard([&] {
-    SHT_REQUIRE(P && (schedule == 0 || schedule == 1), SHT_ERR_INVALID, "schedule must be 0 or 1");
-    if (schedule == 1) {
+    SHT_REQUIRE(P && schedule >= 0 && schedule <= 2, SHT_ERR_INVALID, "schedule must be 0, 1 or 2");
+    if (schedule >= 1) {
       SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED, "A-stationary schedule not available");
       sht::build_as_schedule(P);
     }
@@ -950,19 +1004,28 @@ sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, v
   return guard([&] {
     SHT_REQUIRE(P && reps > 0, SHT_ERR_INVALID, "bad argument");
     hipStream_t s = sht::as_stream(stream);
+    if (ms_out)
+      for (int c = 0; c < 3; ++c) ms_out[c] = -1.f;
     std::vector<int> cand{0};
-    if (sht::as_supported(P)) cand.push_back(1);
-    if (P->prec == SHT_PREC_FP64) cand.push_back(2);  // A-stationary, 4-slot sweeps
+    if (sht::as_supported(P)) {
+      cand.push_back(1);
+      // the fma-chain form has a single A-stationary kernel
+      if (P->prec != SHT_PREC_FP64_EXACT) cand.push_back(2);
+    }
+    if (P->graph) {
+      SHT_HIP(hipGraphExecDestroy(P->graph));
+      SHT_HIP(hipGraphDestroy(P->graph_def));
+      P->graph = nullptr;
+      P->graph_def = nullptr;
+    }
     hipEvent_t e0, e1;
     SHT_HIP(hipEventCreate(&e0));
     SHT_HIP(hipEventCreate(&e1));
     int best = P->schedule;
     float best_ms = 1e30f;
-    int best_variant = P->as_variant;
     for (int c : cand) {
       if (c >= 1) sht::build_as_schedule(P);
-      P->schedule = c >= 1 ? 1 : 0;
-      P->as_variant = c == 2 ? 1 : 0;
+      P->schedule = c;
       sht::enqueue(P, alpha, beta, s);
       SHT_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < reps; ++r) sht::enqueue(P, alpha, beta, s);
@@ -974,14 +1037,12 @@ sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, v
       if (ms_out) ms_out[c] = ms;
       if (ms < best_ms) {
         best_ms = ms;
-        best = c >= 1 ? 1 : 0;
-        best_variant = c == 2 ? 1 : 0;
+        best = c;
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->schedule = best;
-    P->as_variant = best_variant;
   });
 }
 

@@ -59,67 +59,77 @@ int kclass_cols(KClass k);
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
 
 // ---- A-stationary row-panel schedule --------------------------------------
-// A pass = one A block (block row I, contracted key K) with its partners: the
-// B blocks (K, J) whose product lands in C block (I, J), cut into 16-column
-// slots.  A unit = one kAsRows-row chunk of block row I walking all passes of
-// I in ascending K, so every A element is read once per unit and each C tile
-// is updated in place in the task order (C id, K) of the task list.
+// A panel = the B blocks of one contracted key K (the partners of every A
+// block (I, K)) in ascending block column J, converted once per execution
+// into one packed matrix whose columns are the partners' columns side by
+// side.  A pass = one A block (I, K) against the panel of K, swept 16 panel
+// columns (one slot) at a time.  A unit = one chunk of block row I (32 or 64
+// rows) walking all passes of I in ascending K, so each A element is read once
+// per unit and every C element is updated in place in the task order
+// (C id, K) of the task list.
 struct AsPass {
-  int64_t a_off;
-  int32_t k;   // contracted length of the A block
-  int32_t m;   // rows of block row I (= rows of every C block (I, J))
-  int32_t s0;  // first slot
-  int32_t ns;  // slot count
+  int64_t a_off;   // A block (m x k, column-major)
+  int64_t bt_off;  // panel, in elements of the B format
+  int32_t k;       // contracted length of the A block
+  int32_t m;       // rows of block row I (= rows of every C block (I, J))
+  int32_t ncol;    // packed columns of the panel
+  int32_t pitch;   // F64T/F32T: row pitch (ncol rounded up to 16); BF16X3: contracted pad (k rounded up to 32)
+  int32_t col0;    // first entry of the panel's column map
+  int32_t cof0;    // first entry of this pass's partner C offsets
 };
-struct AsSlot {
-  int64_t bt_off;  // offset of the partner's transposed B block (k-major, row pitch npad)
-  int64_t c_off;
-  int32_t n;       // columns of the B / C block
-  int32_t n0;      // first column of this slot
+// Panel column -> (partner index within the panel, column of the partner).
+struct AsCol {
+  int32_t p;
+  int32_t j;
 };
-// B blocks transposed to k-major rows padded to 16 columns (zeros), so a
-// slot's columns for one k are contiguous: coalesced MFMA fragments and
-// wave-uniform scalar loads for the fma-chain kernel.
-constexpr int kBtPad = 16;
-constexpr int kBfK = 32;  // k granularity of the bf16 MFMA (16x16x32)
+constexpr int kBtPad = 16;  // panel columns are padded (zeros) to a multiple of 16
+constexpr int kBfK = 32;    // k granularity of the bf16 MFMA (16x16x32)
+// One record per B block: where its columns go in its panel.
+//  F64T / F32T: element (k, j) -> bt[bt_off + k * pitch + j]; rows up to
+//               roundup(k, 4) are written (zeros past k);
+//  BF16X3: three bf16 planes (hi, mid, lo) of x = hi + mid + lo, bt_total
+//          elements apart; element (k, j) -> plane[bt_off + j * pitch + k],
+//          pitch = roundup(k, 32), zeros past k (so one 16-byte load gives the
+//          8 consecutive k of one column an MFMA lane needs).
+// Pad columns of a panel are zeroed once when the schedule is built.
 struct BtRec {
   int64_t b_off;   // canonical B block (k x n, column-major)
-  int64_t bt_off;  // converted block, in elements of the format
+  int64_t bt_off;  // first column of the block inside its panel
   int32_t k, n;
+  int32_t pitch;
+  int32_t pad;
 };
-// Per-execution B layouts of the A-stationary kernels:
-//  F64T / F32T: k-major rows of npad = roundup(n, 16) columns, roundup(k, 4)
-//               rows (zero padded);
-//  BF16X3: three bf16 planes (hi, mid, lo) of x = hi + mid + lo, each stored
-//          column-major with the contracted dim padded to 32 (zeros), so one
-//          16-byte load gives the 8 consecutive k of one column an MFMA lane
-//          needs.  Planes are bt_total elements apart.
 enum class BtFormat : int { F64T = 0, F32T = 1, BF16X3 = 2 };
 void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const double* B, void* BT, int64_t bt_total,
                hipStream_t s);
 struct AsUnit {
   int32_t pass0;
   int32_t npass;
-  int32_t r0;
+  int32_t r0;    // first row of the chunk
   int32_t pad;
 };
 struct AsArgs {
   const void* A;
-  const void* B;   // converted B (BtFormat of the plan)
+  const void* B;      // panels (BtFormat of the plan)
   void* C;
-  int64_t bt_total;  // elements per bf16 plane
+  int64_t bt_total;   // elements per bf16 plane
   const AsPass* passes;
-  const AsSlot* slots;
+  const AsCol* cols;
+  const int64_t* cofs;  // per pass and partner: C block offset, -1 if the C block is not kept
   const AsUnit* units;
   int64_t nunits;
   double alpha;
   int alpha_one;
 };
-enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64s4 = 2, Mfma32 = 3, Bf16x3 = 4, Count };
-constexpr int kAsRows = 16;     // rows per MFMA unit (one wave); the fma-chain kernel uses 64-row units
-constexpr int kAsSlots = 8;     // 16-column slots per MFMA sweep
-constexpr int kAsxCols = 8;     // columns per fma-chain slot
-constexpr int kAsxSlots = 8;    // slots per fma-chain sweep
+// Kernel classes.  Mfma64/Mfma32/Bf16x3 run 32-row units (two 16-row MFMA
+// tiles per wave), the *W variants 64-row units (four tiles); Exact runs
+// 64-row units with one row per lane.
+enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64W = 2, Mfma32 = 3, Bf16x3 = 4, Mfma32W = 5, Bf16x3W = 6,
+                           Count };
+constexpr int kAsUnitRows = 32;    // rows of a unit of the 2-tile kernels
+constexpr int kAsUnitRowsW = 64;   // rows of a unit of the 4-tile and fma-chain kernels
+constexpr int kAsxCols = 8;        // columns per fma-chain slot
+constexpr int kAsxSlots = 8;       // slots per fma-chain sweep
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
 void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);
@@ -172,18 +182,17 @@ struct sht_plan {
   double bytes_in = 0.0;  // A + B + C write
   double bytes_c = 0.0;   // C read (beta != 0)
   // A-stationary schedule (built on demand)
-  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary row panels
-  int as_variant = 0;  // A-stationary FP64 MFMA kernel: 0 = 4-wave groups, 1 = one-wave lean units
+  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary 32-row units, 2: A-stationary 64-row units
   bool as_built = false;
   sht::AsPass* d_as_pass = nullptr;
-  sht::AsSlot* d_as_part = nullptr;
-  sht::AsUnit* d_as_unit = nullptr;   // 4-wave workgroups (64-row units for the fma chain)
-  sht::AsUnit* d_as_unit1 = nullptr;  // one-wave workgroups
-  int64_t as_nunits1 = 0;
+  sht::AsCol* d_as_col = nullptr;
+  int64_t* d_as_cof = nullptr;
+  sht::AsUnit* d_as_unit = nullptr;   // 32-row units
+  sht::AsUnit* d_as_unitw = nullptr;  // 64-row units
   sht::BtRec* d_bt_rec = nullptr;
   void* d_bt = nullptr;
   int64_t bt_nrec = 0, bt_total = 0;
-  int64_t as_nunits = 0, as_npass = 0, as_npart = 0;
+  int64_t as_nunits = 0, as_nunitsw = 0, as_npass = 0, as_npanel = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   double graph_alpha = 0.0, graph_beta = 0.0;
@@ -194,8 +203,8 @@ struct sht_plan {
     for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
                     static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
-                    static_cast<void*>(d_as_part), static_cast<void*>(d_as_unit), static_cast<void*>(d_as_unit1),
-                    static_cast<void*>(d_bt_rec), d_bt})
+                    static_cast<void*>(d_as_col), static_cast<void*>(d_as_cof), static_cast<void*>(d_as_unit),
+                    static_cast<void*>(d_as_unitw), static_cast<void*>(d_bt_rec), d_bt})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);

@@ -112,7 +112,7 @@ void tensor_reserve_sorted(sht_tensor* t, std::vector<int64_t>&& ids)
   }
   t->nelem = off;
   sht_ctx* c = t->ctx;
-  t->d_data = c->arena.allocate(static_cast<size_t>(std::max<int64_t>(off, 1)) * t->elem_size());
+  t->d_data = c->arena.allocate(static_cast<size_t>(std::max<int64_t>(off, 1)) * t->elem_size() + kReadSlack);
   t->d_indices = static_cast<int64_t*>(c->arena.allocate(std::max<int64_t>(nb, 1) * sizeof(int64_t)));
   t->d_offsets = static_cast<int64_t*>(c->arena.allocate(std::max<int64_t>(nb, 1) * sizeof(int64_t)));
   if (nb) {
