@@ -117,24 +117,27 @@ __device__ inline void sweep_f64(const AsArgs& p, const PassU& ps, int g, int r0
   for (int t = 0; t < R; ++t) arow[t] = min(r0 + 16 * t + li, M - 1);
   const int last = ((K + 3) >> 2) - 1;
   const int64_t bstep = 4 * static_cast<int64_t>(ps.pitch);
+  // Loads fill raw registers; the mask and alpha are applied by the consumer
+  // (after the scheduling barrier), so nothing waits for a load before the
+  // MFMAs that precede its use.
   auto load = [&](int st, double(&a)[R], double(&b)[NS]) {
     const int kk = 4 * st + lk;
     const double* col = A + static_cast<int64_t>(min(kk, K - 1)) * M;
 #pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const double x = col[arow[t]];
-      const double y = kk < K ? x : 0.0;
-      a[t] = p.alpha_one ? y : y * p.alpha;
-    }
+    for (int t = 0; t < R; ++t) a[t] = col[arow[t]];
     const double* bs = Bl + st * bstep;
 #pragma unroll
     for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
   };
-  auto mma = [&](const double(&a)[R], const double(&b)[NS]) {
+  auto mma = [&](int st, const double(&araw)[R], const double(&b)[NS]) {
+    const bool kin = 4 * st + lk < K;
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int t = 0; t < R; ++t) {
+      const double y = kin ? araw[t] : 0.0;
+      const double a = p.alpha_one ? y : y * p.alpha;
 #pragma unroll
-      for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[q], acc[t][q], 0, 0, 0);
+      for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[t][q], 0, 0, 0);
+    }
   };
   double a0[R], a1[R], b0[NS], b1[NS];
   load(0, a0, b0);
@@ -142,14 +145,14 @@ __device__ inline void sweep_f64(const AsArgs& p, const PassU& ps, int g, int r0
   for (; st + 1 <= last; st += 2) {
     load(st + 1, a1, b1);
     __builtin_amdgcn_sched_barrier(0);
-    mma(a0, b0);
+    mma(st, a0, b0);
     __builtin_amdgcn_sched_barrier(0);
     load(min(st + 2, last), a0, b0);
     __builtin_amdgcn_sched_barrier(0);
-    mma(a1, b1);
+    mma(st + 1, a1, b1);
     __builtin_amdgcn_sched_barrier(0);
   }
-  if (st == last) mma(a0, b0);  // odd step count: a0/b0 hold the last step
+  if (st == last) mma(st, a0, b0);  // odd step count: a0/b0 hold the last step
 #pragma unroll
   for (int q = 0; q < NS; ++q) {
 #pragma unroll
@@ -214,38 +217,39 @@ __device__ inline void sweep_f32(const AsArgs& p, const PassU& ps, int g, int r0
   for (int t = 0; t < R; ++t) arow[t] = min(r0 + 16 * t + li, M - 1);
   const int last = ((K + 3) >> 2) - 1;
   const int64_t bstep = 4 * static_cast<int64_t>(ps.pitch);
-  auto load = [&](int st, float(&a)[R], float(&b)[NS]) {
+  auto load = [&](int st, double(&a)[R], float(&b)[NS]) {
     const int kk = 4 * st + lk;
     const double* col = A + static_cast<int64_t>(min(kk, K - 1)) * M;
 #pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const float x = static_cast<float>(col[arow[t]]);
-      a[t] = kk < K ? x : 0.f;
-    }
+    for (int t = 0; t < R; ++t) a[t] = col[arow[t]];
     const float* bs = Bl + st * bstep;
 #pragma unroll
     for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
   };
-  auto mma = [&](const float(&a)[R], const float(&b)[NS]) {
+  auto mma = [&](int st, const double(&araw)[R], const float(&b)[NS]) {
+    const bool kin = 4 * st + lk < K;
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int t = 0; t < R; ++t) {
+      const float a = kin ? static_cast<float>(araw[t]) : 0.f;
 #pragma unroll
-      for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[q], acc[t][q], 0, 0, 0);
+      for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[q], acc[t][q], 0, 0, 0);
+    }
   };
-  float a0[R], a1[R], b0[NS], b1[NS];
+  double a0[R], a1[R];
+  float b0[NS], b1[NS];
   load(0, a0, b0);
   int st = 0;
   for (; st + 1 <= last; st += 2) {
     load(st + 1, a1, b1);
     __builtin_amdgcn_sched_barrier(0);
-    mma(a0, b0);
+    mma(st, a0, b0);
     __builtin_amdgcn_sched_barrier(0);
     load(min(st + 2, last), a0, b0);
     __builtin_amdgcn_sched_barrier(0);
-    mma(a1, b1);
+    mma(st + 1, a1, b1);
     __builtin_amdgcn_sched_barrier(0);
   }
-  if (st == last) mma(a0, b0);  // odd step count: a0/b0 hold the last step
+  if (st == last) mma(st, a0, b0);  // odd step count: a0/b0 hold the last step
   as_lowp_epilogue<R, NS>(p, ps, g, r0, lk, li, acc);
 }
 
