@@ -24,6 +24,8 @@
 //    per block product, FP64 across blocks.
 //  * k_as_exact: the fma chain with one row per lane (bitwise as well).
 
+#include <type_traits>
+
 #include "plan.hpp"
 
 namespace sht {
@@ -76,181 +78,313 @@ __device__ inline int64_t col_base(const AsArgs& p, const PassU& ps, int c)
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
 // ---------------------------------------------------------------------------
-// FP64 MFMA sweep: R row tiles x NS slots.  The k loop is unrolled by two
-// with two register sets, so the loads of step s+1 are in flight while the
-// MFMAs of step s issue; every branch in it is wave-uniform.  Step indexes
-// are clamped to the last step (every load in bounds and unconditional), A
-// rows past M read row M-1 (their outputs are never stored), A columns past K
-// read column K-1 masked to zero, and panel rows past K are zero.
+// FP64 / FP32 MFMA sweeps: R row tiles x NS slots, four k-steps (16 k) per
+// loop iteration (panel rows are zero-padded to a multiple of 16).
+//
+// The A operand of a unit is one stream across all its sweeps: a ring of
+// four register sets holds the A fragments of the next steps, filled three
+// steps ahead, and the last iteration of a sweep already loads the first
+// three steps of the next sweep (same pass, next slot group; or the next
+// pass), so the A stream from HBM does not restart at sweep boundaries.  B
+// is loaded one step ahead into two register sets.  Loads fill raw
+// registers; the k < K mask and alpha are applied by the consumer after the
+// scheduling barrier, so nothing waits for a load before the MFMAs that
+// precede its use.  Every load is in bounds and unconditional: step indexes
+// are clamped, A rows past M read on into the next column (tensor buffers
+// carry kReadSlack bytes of slack), A columns past K re-read column K-1
+// (masked to zero), and panel rows past K are zero.
 // ---------------------------------------------------------------------------
-template <int R, int NS>
-__device__ inline void sweep_f64(const AsArgs& p, const PassU& ps, int g, int r0, int nt, int li, int lk)
+struct AStream {
+  const double* a;  // A block (wave-uniform)
+  int K, M;
+};
+
+__device__ inline AStream a_stream(const AsArgs& p, const PassU& ps)
 {
-  const double* A = static_cast<const double*>(p.A) + ps.a_off;
-  const double* Bl = static_cast<const double*>(p.B) + ps.bt_off + 16 * g + li + static_cast<int64_t>(lk) * ps.pitch;
-  double* C = static_cast<double*>(p.C);
-  const int K = ps.k, M = ps.m;
-  int64_t cb[NS];
-  v4d acc[R][NS];
+  AStream s;
+  s.a = static_cast<const double*>(p.A) + ps.a_off;
+  s.K = ps.k;
+  s.M = ps.m;
+  return s;
+}
+
+// A fragments of step st of the stream (rows r0 + 16t + li, column 4st + lk)
+template <int R>
+__device__ inline void load_a(const AStream& s, int st, int row, int lk, double (&a)[R])
+{
+  const double* col = s.a + static_cast<int64_t>(min(4 * st + lk, s.K - 1)) * s.M + row;
 #pragma unroll
-  for (int q = 0; q < NS; ++q) cb[q] = col_base(p, ps, 16 * (g + q) + li);
-  // Accumulators start from C.  Pad columns read column 0 of C and rows past
-  // M read on into the following memory (tensor buffers carry kReadSlack
-  // bytes of slack); those results are never stored.  Each slot's values
-  // pass through a 0*0 + C MFMA as soon as they arrive, which moves them into
-  // accumulation registers (D = C exactly, except that -0 becomes +0), so the
-  // staging registers of only two slots are live at a time.
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    const double* cq = C + (cb[q] >= 0 ? cb[q] : 0) + r0 + lk;
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      v4d c;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) c[v] = cq[16 * t + 4 * v];
-      acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, c, 0, 0, 0);
-    }
-    if (q % 2 == 1) __builtin_amdgcn_sched_barrier(0);
-  }
-  int arow[R];
-#pragma unroll
-  for (int t = 0; t < R; ++t) arow[t] = min(r0 + 16 * t + li, M - 1);
-  const int last = ((K + 3) >> 2) - 1;
+  for (int t = 0; t < R; ++t) a[t] = col[16 * t];
+}
+
+// Step st of the current sweep's stream when st < nsteps, else step
+// st - nsteps of the next sweep's (the selects are wave-uniform).
+template <int R>
+__device__ inline void load_a2(const AStream& cu, const AStream& nx, int nsteps, int st, int row, int lk,
+                               double (&a)[R])
+{
+  const bool n = st >= nsteps;
+  AStream s;
+  s.a = n ? nx.a : cu.a;
+  s.K = n ? nx.K : cu.K;
+  s.M = n ? nx.M : cu.M;
+  load_a(s, n ? st - nsteps : st, row, lk, a);
+}
+
+template <int F> struct BElem { typedef double type; };
+template <> struct BElem<1> { typedef float type; };
+
+// One sweep.  F = 0: FP64 MFMA, accumulators start from C (each C element is
+// one fma chain in k order); F = 1: FP32 MFMA from zero, then added into C in
+// FP64.  `raw_next` is the record of the pass after ps (loaded by the caller
+// before this sweep); `next_pass` says whether the next sweep belongs to it.
+// Returns the pass of the next sweep.
+//
+// The MFMA computes the transposed tile, D = Panel^T-fragment x A^T-fragment
+// (products commute, so every element is the same fma chain): lane l then
+// holds rows r0 + 16t + l%16 of four panel columns, so each C load and store
+// instruction covers four columns x 16 consecutive rows (128 contiguous bytes
+// per column).  The C offsets of the sweep's columns are staged in LDS.
+template <int R, int NS, int F>
+__device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int r0, int li, int lk,
+                                   double (&ar)[4][R], const AsPass& raw_next, bool next_pass, int64_t* cbs)
+{
+  typedef typename BElem<F>::type TB;
+  typedef typename std::conditional<F == 0, v4d, v4f>::type Acc;
+  const TB* Bl = static_cast<const TB*>(p.B) + ps.bt_off + 16 * g + li + static_cast<int64_t>(lk) * ps.pitch;
   const int64_t bstep = 4 * static_cast<int64_t>(ps.pitch);
-  // Loads fill raw registers; the mask and alpha are applied by the consumer
-  // (after the scheduling barrier), so nothing waits for a load before the
-  // MFMAs that precede its use.
-  auto load = [&](int st, double(&a)[R], double(&b)[NS]) {
-    const int kk = 4 * st + lk;
-    const double* col = A + static_cast<int64_t>(min(kk, K - 1)) * M;
-#pragma unroll
-    for (int t = 0; t < R; ++t) a[t] = col[arow[t]];
-    const double* bs = Bl + st * bstep;
-#pragma unroll
-    for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
-  };
-  auto mma = [&](int st, const double(&araw)[R], const double(&b)[NS]) {
-    const bool kin = 4 * st + lk < K;
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const double y = kin ? araw[t] : 0.0;
-      const double a = p.alpha_one ? y : y * p.alpha;
-#pragma unroll
-      for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[t][q], 0, 0, 0);
-    }
-  };
-  double a0[R], a1[R], b0[NS], b1[NS];
-  load(0, a0, b0);
-  int st = 0;
-  for (; st + 1 <= last; st += 2) {
-    load(st + 1, a1, b1);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(st, a0, b0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(min(st + 2, last), a0, b0);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(st + 1, a1, b1);
-    __builtin_amdgcn_sched_barrier(0);
+  const int K = ps.k;
+  double* C = static_cast<double*>(p.C);
+  // column c of the sweep held by this lane for accumulator element v
+  auto colof = [&](int q, int v) { return F == 0 ? 16 * q + 4 * v + lk : 16 * q + 4 * lk + v; };
+  {
+    const int lane = threadIdx.x;
+    if (lane < 16 * NS) cbs[lane] = col_base(p, ps, 16 * g + lane);
+    if (lane + 64 < 16 * NS) cbs[lane + 64] = col_base(p, ps, 16 * g + lane + 64);
+    __syncthreads();
   }
-  if (st == last) mma(st, a0, b0);  // odd step count: a0/b0 hold the last step
+  const int rl = r0 + li;  // this lane's row in tile 0
+  Acc acc[R][NS];
+  if constexpr (F == 0) {
+    // Accumulators start from C, all loads in flight at once (straight into
+    // accumulation registers).  Pad columns read column 0 of C, rows past M
+    // read on into the buffer slack; never stored.
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t cb = cbs[colof(q, v)];
+        const double* cq = C + (cb >= 0 ? cb : 0) + rl;
+#pragma unroll
+        for (int t = 0; t < R; ++t) acc[t][q][v] = cq[16 * t];
+      }
+  } else {
 #pragma unroll
     for (int t = 0; t < R; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = r0 + 16 * t + 4 * v + lk;
-        if (cb[q] >= 0 && row < M) C[cb[q] + row] = acc[t][q][v];
+      for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  TB b0[NS], b1[NS];
+  auto loadB = [&](int st, TB(&b)[NS]) {
+    const TB* bs = Bl + st * bstep;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
+  };
+  loadB(0, b0);
+  __builtin_amdgcn_sched_barrier(0);
+  // the next sweep's A stream (its record has arrived by now)
+  PassU nps = ps;
+  if (next_pass) {
+    nps.a_off = uni(raw_next.a_off);
+    nps.bt_off = uni(raw_next.bt_off);
+    nps.k = uni(raw_next.k);
+    nps.m = uni(raw_next.m);
+    nps.ncol = uni(raw_next.ncol);
+    nps.pitch = uni(raw_next.pitch);
+    nps.col0 = uni(raw_next.col0);
+    nps.cof0 = uni(raw_next.cof0);
+  }
+  const AStream cu = a_stream(p, ps), nx = a_stream(p, nps);
+  auto mma = [&](int st, const double(&araw)[R], const TB(&b)[NS]) {
+    const bool kin = 4 * st + lk < K;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if constexpr (F == 0) {
+        const double y = kin ? araw[t] : 0.0;
+        const double a = p.alpha_one ? y : y * p.alpha;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a, acc[t][q], 0, 0, 0);
+      } else {
+        const float a = kin ? static_cast<float>(araw[t]) : 0.f;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[q], a, acc[t][q], 0, 0, 0);
       }
+    }
+  };
+  // B indexes are clamped to the last step; A indexes past it continue into
+  // the next sweep's stream (the ring then holds its steps 0..2).  In each
+  // phase the B loads (one step ahead, mostly L2 hits) are issued before the
+  // A loads (three steps ahead, from HBM): the vector memory counter retires
+  // in issue order, so waiting for B does not wait for the younger A loads.
+  const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
+  const int row = r0 + li;
+  for (int st = 0; st < nsteps; st += 4) {
+    loadB(st + 1, b1);
+    load_a2(cu, nx, nsteps, st + 3, row, lk, ar[3]);
     __builtin_amdgcn_sched_barrier(0);
+    mma(st, ar[0], b0);
+    __builtin_amdgcn_sched_barrier(0);
+    loadB(min(st + 2, last), b0);
+    load_a2(cu, nx, nsteps, st + 4, row, lk, ar[0]);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(st + 1, ar[1], b1);
+    __builtin_amdgcn_sched_barrier(0);
+    loadB(min(st + 3, last), b1);
+    load_a2(cu, nx, nsteps, st + 5, row, lk, ar[1]);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(st + 2, ar[2], b0);
+    __builtin_amdgcn_sched_barrier(0);
+    loadB(min(st + 4, last), b0);
+    load_a2(cu, nx, nsteps, st + 6, row, lk, ar[2]);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(st + 3, ar[3], b1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // write back
+  const int M = ps.m;
+  if constexpr (F == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t cb = cbs[colof(q, v)];
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+          if (cb >= 0 && rl + 16 * t < M) C[cb + rl + 16 * t] = acc[t][q][v];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    // C += alpha * (FP32 block product) in FP64, 4/R slots at a time (16
+    // doubles per lane in flight).
+    constexpr int G = NS < 4 / R ? NS : (4 / R > 0 ? 4 / R : 1);
+#pragma unroll
+    for (int q0 = 0; q0 < NS; q0 += G) {
+      int64_t cb[G][4];
+      double cv[G][4][R];
+#pragma unroll
+      for (int x = 0; x < G; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          cb[x][v] = q0 + x < NS ? cbs[colof(q0 + x, v)] : -1;
+          const double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
+#pragma unroll
+          for (int t = 0; t < R; ++t) cv[x][v][t] = cq[16 * t];
+        }
+#pragma unroll
+      for (int x = 0; x < G; ++x) {
+        if (q0 + x >= NS) break;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
+#pragma unroll
+          for (int t = 0; t < R; ++t)
+            if (cb[x][v] >= 0 && rl + 16 * t < M)
+              cq[16 * t] = cv[x][v][t] + p.alpha * static_cast<double>(acc[t][q0 + x][v]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __syncthreads();  // the column offsets are rewritten by the next sweep
+  return nps;
+}
+
+// F: 0 = FP64 MFMA, 1 = FP32 MFMA.  One wave per unit of 16*R rows; sweeps
+// of up to NSMAX slots.
+template <int R, int NSMAX, int F>
+__global__ __launch_bounds__(64) void k_asp(AsArgs p)
+{
+  if (static_cast<int64_t>(blockIdx.x) >= p.nunits) return;
+  const AsUnit u = p.units[blockIdx.x];
+  const int npass = uni(u.npass), pass0 = uni(u.pass0), r0 = uni(u.r0);
+  if (npass == 0) return;
+  const int li = threadIdx.x & 15, lk = threadIdx.x >> 4;
+  const int pend = pass0 + npass;
+  __shared__ int64_t cbs[16 * NSMAX];
+  PassU ps = load_pass(p.passes, pass0);
+  double ar[4][R];
+  {
+    const AStream s0 = a_stream(p, ps);
+    load_a(s0, 0, r0 + li, lk, ar[0]);
+    load_a(s0, 1, r0 + li, lk, ar[1]);
+    load_a(s0, 2, r0 + li, lk, ar[2]);
+  }
+  int pi = pass0, g = 0;
+  while (true) {
+    const int nslots = (ps.ncol + 15) >> 4;
+    const bool more_groups = g + NSMAX < nslots;
+    const bool next_pass = !more_groups && pi + 1 < pend;
+    const AsPass raw_next = p.passes[min(pi + 1, pend - 1)];
+    PassU nps;
+    switch (min(NSMAX, nslots - g)) {
+      case 1: nps = sweep_pipe<R, 1, F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      case 2: nps = sweep_pipe<R, cmin(2, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      case 3: nps = sweep_pipe<R, cmin(3, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      case 4: nps = sweep_pipe<R, cmin(4, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      case 5: nps = sweep_pipe<R, cmin(5, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      case 6: nps = sweep_pipe<R, cmin(6, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      case 7: nps = sweep_pipe<R, cmin(7, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+      default: nps = sweep_pipe<R, cmin(8, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
+    }
+    if (more_groups) {
+      g += NSMAX;
+    } else if (next_pass) {
+      ps = nps;
+      ++pi;
+      g = 0;
+    } else {
+      break;
+    }
   }
 }
 
-// ---------------------------------------------------------------------------
-// C += alpha * (FP32 block product) in FP64, one slot at a time.  The f32
-// accumulator layout puts rows 4*(l/16) + v of column l%16 in lane l.  Loads
-// are unconditional (rows past M read into the buffer slack), stores are
-// guarded.
-// ---------------------------------------------------------------------------
+// C += alpha * (FP32 block product) in FP64 for the split-BF16 sweep, 8/R
+// slots at a time (32 doubles per lane in flight).  The f32 accumulator
+// layout puts rows 4*(l/16) + v of column l%16 in lane l.
 template <int R, int NS>
 __device__ inline void as_lowp_epilogue(const AsArgs& p, const PassU& ps, int g, int r0, int lk, int li,
                                         const v4f (&acc)[R][NS])
 {
+  constexpr int G = NS < 8 / R ? NS : 8 / R;
   double* C = static_cast<double*>(p.C);
   const int M = ps.m;
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    const int64_t cb = col_base(p, ps, 16 * (g + q) + li);
-    double* cq = C + (cb >= 0 ? cb : 0) + r0 + 4 * lk;
-    double cv[R][4];
+  for (int q0 = 0; q0 < NS; q0 += G) {
+    int64_t cb[G];
+    double cv[G][R][4];
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int x = 0; x < G; ++x) {
+      cb[x] = q0 + x < NS ? col_base(p, ps, 16 * (g + q0 + x) + li) : -1;
+      const double* cq = C + (cb[x] >= 0 ? cb[x] : 0) + r0 + 4 * lk;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) cv[t][v] = cq[16 * t + v];
+      for (int t = 0; t < R; ++t)
 #pragma unroll
-    for (int t = 0; t < R; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (cb >= 0 && r0 + 16 * t + 4 * lk + v < M) cq[16 * t + v] = cv[t][v] + p.alpha * static_cast<double>(acc[t][q][v]);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// FP32 MFMA sweep: same pipeline; A converted to FP32 on load, the panel is
-// FP32; the block product accumulates in FP32 and is added into C in FP64.
-// ---------------------------------------------------------------------------
-template <int R, int NS>
-__device__ inline void sweep_f32(const AsArgs& p, const PassU& ps, int g, int r0, int nt, int li, int lk)
-{
-  const double* A = static_cast<const double*>(p.A) + ps.a_off;
-  const float* Bl = static_cast<const float*>(p.B) + ps.bt_off + 16 * g + li + static_cast<int64_t>(lk) * ps.pitch;
-  double* C = static_cast<double*>(p.C);
-  const int K = ps.k, M = ps.m;
-  v4f acc[R][NS];
-#pragma unroll
-  for (int t = 0; t < R; ++t)
-#pragma unroll
-    for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
-  int arow[R];
-#pragma unroll
-  for (int t = 0; t < R; ++t) arow[t] = min(r0 + 16 * t + li, M - 1);
-  const int last = ((K + 3) >> 2) - 1;
-  const int64_t bstep = 4 * static_cast<int64_t>(ps.pitch);
-  auto load = [&](int st, double(&a)[R], float(&b)[NS]) {
-    const int kk = 4 * st + lk;
-    const double* col = A + static_cast<int64_t>(min(kk, K - 1)) * M;
-#pragma unroll
-    for (int t = 0; t < R; ++t) a[t] = col[arow[t]];
-    const float* bs = Bl + st * bstep;
-#pragma unroll
-    for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
-  };
-  auto mma = [&](int st, const double(&araw)[R], const float(&b)[NS]) {
-    const bool kin = 4 * st + lk < K;
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const float a = kin ? static_cast<float>(araw[t]) : 0.f;
-#pragma unroll
-      for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[q], acc[t][q], 0, 0, 0);
+        for (int v = 0; v < 4; ++v) cv[x][t][v] = cq[16 * t + v];
     }
-  };
-  double a0[R], a1[R];
-  float b0[NS], b1[NS];
-  load(0, a0, b0);
-  int st = 0;
-  for (; st + 1 <= last; st += 2) {
-    load(st + 1, a1, b1);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(st, a0, b0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(min(st + 2, last), a0, b0);
-    __builtin_amdgcn_sched_barrier(0);
-    mma(st + 1, a1, b1);
+#pragma unroll
+    for (int x = 0; x < G; ++x) {
+      if (q0 + x >= NS) break;
+      double* cq = C + (cb[x] >= 0 ? cb[x] : 0) + r0 + 4 * lk;
+#pragma unroll
+      for (int t = 0; t < R; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (cb[x] >= 0 && r0 + 16 * t + 4 * lk + v < M)
+            cq[16 * t + v] = cv[x][t][v] + p.alpha * static_cast<double>(acc[t][q0 + x][v]);
+    }
     __builtin_amdgcn_sched_barrier(0);
   }
-  if (st == last) mma(st, a0, b0);  // odd step count: a0/b0 hold the last step
-  as_lowp_epilogue<R, NS>(p, ps, g, r0, lk, li, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,13 +479,11 @@ __device__ inline void sweep_bf3(const AsArgs& p, const PassU& ps, int g, int r0
 template <int R, int F, int NS>
 __device__ inline void run_sweep(const AsArgs& p, const PassU& ps, int g, int r0, int nt, int li, int lk)
 {
-  if constexpr (F == 0) sweep_f64<R, NS>(p, ps, g, r0, nt, li, lk);
-  else if constexpr (F == 1) sweep_f32<R, NS>(p, ps, g, r0, nt, li, lk);
-  else sweep_bf3<R, NS>(p, ps, g, r0, nt, li, lk);
+  static_assert(F == 2, "the MFMA forms run k_asp");
+  sweep_bf3<R, NS>(p, ps, g, r0, nt, li, lk);
 }
 
-// F: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = split-BF16.  One wave per unit of
-// 16*R rows; sweeps of up to NSMAX slots.
+// Split-BF16: one wave per unit of 16*R rows; sweeps of up to NSMAX slots.
 template <int R, int NSMAX, int F>
 __global__ __launch_bounds__(64) void k_as(AsArgs p)
 {
@@ -442,7 +574,7 @@ __global__ __launch_bounds__(256) void k_bt(const BtRec* recs, int64_t nrec, con
 {
   for (int64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
     const BtRec rec = recs[r];
-    const int krows = (rec.k + 3) / 4 * 4;  // zero rows up to a multiple of 4
+    const int krows = (rec.k + 15) / 16 * 16;  // zero rows up to a multiple of 16
     const int64_t tot = static_cast<int64_t>(krows) * rec.n;
     const double* b = B + rec.b_off;
     T* o = BT + rec.bt_off;
@@ -483,16 +615,25 @@ __global__ __launch_bounds__(256) void k_scale(double* c, int64_t n, double beta
 
 }  // namespace
 
+int as_unit_rows(AsClass k)
+{
+  switch (k) {
+    case AsClass::Exact:
+    case AsClass::Bf16x3W: return kAsUnitRowsW;
+    default: return kAsUnitRows;
+  }
+}
+
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
 {
   if (a.nunits <= 0) return;
   const dim3 g(static_cast<unsigned>(a.nunits)), b(64);
   static_assert(kAsUnitRows == 32 && kAsUnitRowsW == 64, "unit rows: 2 and 4 MFMA tiles");
   switch (k) {
-    case AsClass::Mfma64: hipLaunchKernelGGL((k_as<2, 8, 0>), g, b, 0, s, a); break;
-    case AsClass::Mfma64W: hipLaunchKernelGGL((k_as<4, 4, 0>), g, b, 0, s, a); break;
-    case AsClass::Mfma32: hipLaunchKernelGGL((k_as<2, 8, 1>), g, b, 0, s, a); break;
-    case AsClass::Mfma32W: hipLaunchKernelGGL((k_as<4, 4, 1>), g, b, 0, s, a); break;
+    case AsClass::Mfma64: hipLaunchKernelGGL((k_asp<2, 8, 0>), g, b, 0, s, a); break;
+    case AsClass::Mfma64W: hipLaunchKernelGGL((k_asp<2, 4, 0>), g, b, 0, s, a); break;
+    case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1>), g, b, 0, s, a); break;
+    case AsClass::Mfma32W: hipLaunchKernelGGL((k_asp<2, 4, 1>), g, b, 0, s, a); break;
     case AsClass::Bf16x3: hipLaunchKernelGGL((k_as<2, 4, 2>), g, b, 0, s, a); break;
     case AsClass::Bf16x3W: hipLaunchKernelGGL((k_as<4, 4, 2>), g, b, 0, s, a); break;
     case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g, b, 0, s, a); break;

@@ -611,7 +611,7 @@ void build_as_schedule(sht_plan* P)
         bt.push_back(BtRec{B->offsets[q], pn.bt_off + c, pn.k, n, pn.pitch, 0});
         c += n;
       }
-      bt_total += static_cast<int64_t>((pn.k + 3) / 4 * 4) * npad;
+      bt_total += static_cast<int64_t>((pn.k + 15) / 16 * 16) * npad;
     }
     panels.push_back(pn);
     u = e;
@@ -808,7 +808,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.cols = P->d_as_col;
       g.cofs = P->d_as_cof;
       const AsClass cls = as_class(P);
-      const bool wide = cls == AsClass::Exact || P->schedule == 2;
+      const bool wide = as_unit_rows(cls) == kAsUnitRowsW;
       g.units = wide ? P->d_as_unitw : P->d_as_unit;
       g.nunits = wide ? P->as_nunitsw : P->as_nunits;
       g.alpha = alpha;

@@ -86,7 +86,7 @@ constexpr int kBtPad = 16;  // panel columns are padded (zeros) to a multiple of
 constexpr int kBfK = 32;    // k granularity of the bf16 MFMA (16x16x32)
 // One record per B block: where its columns go in its panel.
 //  F64T / F32T: element (k, j) -> bt[bt_off + k * pitch + j]; rows up to
-//               roundup(k, 4) are written (zeros past k);
+//               roundup(k, 16) are written (zeros past k);
 //  BF16X3: three bf16 planes (hi, mid, lo) of x = hi + mid + lo, bt_total
 //          elements apart; element (k, j) -> plane[bt_off + j * pitch + k],
 //          pitch = roundup(k, 32), zeros past k (so one 16-byte load gives the
@@ -131,6 +131,8 @@ constexpr int kAsUnitRowsW = 64;   // rows of a unit of the 4-tile and fma-chain
 constexpr int kAsxCols = 8;        // columns per fma-chain slot
 constexpr int kAsxSlots = 8;       // slots per fma-chain sweep
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
+// rows of the units a class runs (kAsUnitRows or kAsUnitRowsW)
+int as_unit_rows(AsClass k);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
 void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);
 
