@@ -109,12 +109,14 @@ __device__ inline AStream a_stream(const AsArgs& p, const PassU& ps)
 }
 
 // A fragments of step st of the stream (rows r0 + 16t + li, column 4st + lk)
+// A is read once per unit: non-temporal loads keep the stream from evicting
+// the panels and C tiles that the other units of the block row reuse in L2.
 template <int R>
 __device__ inline void load_a(const AStream& s, int st, int row, int lk, double (&a)[R])
 {
   const double* col = s.a + static_cast<int64_t>(min(4 * st + lk, s.K - 1)) * s.M + row;
 #pragma unroll
-  for (int t = 0; t < R; ++t) a[t] = col[16 * t];
+  for (int t = 0; t < R; ++t) a[t] = __builtin_nontemporal_load(col + 16 * t);
 }
 
 // Step st of the current sweep's stream when st < nsteps, else step
@@ -184,13 +186,16 @@ __device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int 
 #pragma unroll
       for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
   }
-  TB b0[NS], b1[NS];
+  TB br[4][NS];
   auto loadB = [&](int st, TB(&b)[NS]) {
     const TB* bs = Bl + st * bstep;
 #pragma unroll
     for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
   };
-  loadB(0, b0);
+  const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
+  loadB(0, br[0]);
+  loadB(1, br[1]);
+  loadB(2, br[2]);
   __builtin_amdgcn_sched_barrier(0);
   // the next sweep's A stream (its record has arrived by now)
   PassU nps = ps;
@@ -221,33 +226,32 @@ __device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int 
       }
     }
   };
-  // B indexes are clamped to the last step; A indexes past it continue into
-  // the next sweep's stream (the ring then holds its steps 0..2).  In each
-  // phase the B loads (one step ahead, mostly L2 hits) are issued before the
-  // A loads (three steps ahead, from HBM): the vector memory counter retires
-  // in issue order, so waiting for B does not wait for the younger A loads.
-  const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
+  // Both operands are loaded three steps ahead into rings of four register
+  // sets.  B indexes are clamped to the last step; A indexes past it continue
+  // into the next sweep's stream (the A ring then holds its steps 0..2).  In
+  // each phase the B loads (mostly L2 hits) are issued before the A loads
+  // (from HBM): the vector memory counter retires in issue order.
   const int row = r0 + li;
   for (int st = 0; st < nsteps; st += 4) {
-    loadB(st + 1, b1);
+    loadB(min(st + 3, last), br[3]);
     load_a2(cu, nx, nsteps, st + 3, row, lk, ar[3]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st, ar[0], b0);
+    mma(st, ar[0], br[0]);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(min(st + 2, last), b0);
+    loadB(min(st + 4, last), br[0]);
     load_a2(cu, nx, nsteps, st + 4, row, lk, ar[0]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st + 1, ar[1], b1);
+    mma(st + 1, ar[1], br[1]);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(min(st + 3, last), b1);
+    loadB(min(st + 5, last), br[1]);
     load_a2(cu, nx, nsteps, st + 5, row, lk, ar[1]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st + 2, ar[2], b0);
+    mma(st + 2, ar[2], br[2]);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(min(st + 4, last), b0);
+    loadB(min(st + 6, last), br[2]);
     load_a2(cu, nx, nsteps, st + 6, row, lk, ar[2]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st + 3, ar[3], b1);
+    mma(st + 3, ar[3], br[3]);
     __builtin_amdgcn_sched_barrier(0);
   }
   // write back
