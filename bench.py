@@ -31,7 +31,18 @@ PEAK_BY_PREC = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15 / 6}
 
 def peak_of(form):
     return PEAK_BY_PREC[form.split("_")[0]]
+
+
 HBM_PEAK = 8.0e12
+# kernel class reported by sht_plan_execute_timed -> substring of the kernel's
+# symbol in rocprofv3's output
+KERNEL_SYMBOL = {
+    "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asp<2, 8, 0, 4, true>",
+    "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asp<2, 8, 1, 4, false>", "as_bf16x3": "k_as<2, 4, 2, 1>", "as_bf16x3_g": "k_as<2, 4, 2, 4>",
+    "as_exact": "k_as_exact", "mfma64_n16": "k_mfma64<1>", "mfma64_n32": "k_mfma64<2>",
+    "mfma32_n16": "k_mfma32<1>", "mfma32_n32": "k_mfma32<2>", "bf16x3_n16": "k_bf16x3<1>",
+    "bf16x3_n32": "k_bf16x3<2>", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
+}
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
 
 CONFIGS = {
@@ -135,6 +146,60 @@ def pick_sample(cfg, sizes, eabc, orc, target_flops):
     return As, B, Cs, nrows, float(cum[nrows - 1])
 
 
+def measure_traffic(args, form, schedule, kclass, c_nelem):
+    """HBM bytes per launch of the dominant kernel: two child runs of this
+    script under `rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE; KiB per
+    dispatch), each scaled by what the same counter reads for a plain
+    read+write pass of known size (C = 1*C over C's elements, the --calib
+    pass).  Runs after this process has released its device memory."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    rp = shutil.which("rocprofv3")
+    sym = KERNEL_SYMBOL.get(kclass)
+    if not rp or not sym:
+        return None
+    res = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="sht_pmc_", dir="/tmp")
+        cmd = [rp, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.join(ROOT, "bench.py"), "--config", args.config, "--forms", form, "--steps", "1",
+               "--warmup", "0", "--no-cpu", "--no-traffic", "--calib", "--schedule", str(schedule)]
+        if args.length:
+            cmd += ["--length", str(args.length)]
+        log("traffic: %s" % " ".join(cmd))
+        try:
+            subprocess.run(cmd, cwd=d, env=env, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        except subprocess.TimeoutExpired:
+            log("traffic: %s run timed out" % ctr)
+            return None
+        files = glob.glob(os.path.join(d, "**", "run_counter_collection.csv"), recursive=True)
+        if not files:
+            log("traffic: no counter output for %s" % ctr)
+            return None
+        kern, calib = [], []
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != ctr:
+                    continue
+                v = float(row["Counter_Value"]) * 1024.0
+                if sym in row["Kernel_Name"]:
+                    kern.append(v)
+                elif "k_scale" in row["Kernel_Name"]:
+                    calib.append(v)
+        shutil.rmtree(d, ignore_errors=True)
+        if not kern or not calib or max(calib) <= 0:
+            log("traffic: kernel or calibration dispatch missing for %s" % ctr)
+            return None
+        factor = 8.0 * c_nelem / max(calib)
+        res[ctr] = {"measured": float(np.mean(kern)), "factor": factor, "bytes": float(np.mean(kern)) * factor}
+    res["bytes"] = res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +213,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="force the execution schedule (0, 1, 2) instead of the form's default/autotuned one")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs that measure the dominant kernel's HBM bytes")
     ap.add_argument("--graph", action="store_true", help="time hipGraph replays instead of eager steps")
     ap.add_argument("--calib", action="store_true",
                     help="also run C = 1*C (alpha=0) through the A-stationary path: a plain read+write of C "
@@ -181,7 +250,9 @@ def main():
         tp = time.time()
         plan = sht.Plan(ctx, expr, A, B, Cf, precs[f])
         tuned = None
-        if f in ("fp64", "fp64_exact", "fp32", "bf16x3"):
+        if args.schedule >= 0:
+            plan.set_schedule(args.schedule)
+        elif f in ("fp64", "fp64_exact", "fp32", "bf16x3"):
             tuned = plan.autotune(1.0, 0.0, reps=2, stream=stream)  # picks the schedule
         elif f.endswith("_as"):
             plan.set_schedule(1)
@@ -208,9 +279,13 @@ def main():
             stream.synchronize()
             times.append(time.perf_counter() - ts)
         timed = plan.execute_timed(1.0, 0.0, stream)
-        if args.calib and plan.get_schedule() == 1:
+        if args.calib:
+            sched = plan.get_schedule()
+            if sched == 0:
+                plan.set_schedule(1)  # the A-stationary path starts with C = beta*C (k_scale)
             plan.execute(0.0, 1.0, stream)  # k_scale only: reads and writes every C element once
             stream.synchronize()
+            plan.set_schedule(sched)
             log("calibration: C = 1*C over %d elements (%.3e bytes read, same written)" % (Cf.nelem(),
                                                                                          8.0 * Cf.nelem()))
         times = np.array(times)
@@ -302,10 +377,18 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    c_nelem = Ctens[main_f].nelem()
     for f in forms:
         results[f]["plan"].close()
     ctx.close()
+    if not args.no_traffic:
+        tr = measure_traffic(args, main_f, r["stats"]["schedule"], dom[0], c_nelem)
+        if tr:
+            roof["traffic"] = round(tr["bytes"], 0)
+            roof["traffic_detail"] = {k: {"measured_bytes": round(v["measured"], 0), "calibration": round(v["factor"], 4)}
+                                      for k, v in tr.items() if k != "bytes"}
+            roof["algorithmic_bytes"] = r["stats"].get("bytes")
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,27 +1,32 @@
 // A-stationary row-panel kernels of a contraction plan.
 //
-// A unit (one wave, one workgroup) owns a chunk of rows of block row I and
-// walks the passes of I: the A blocks (I, K) in ascending contracted key K.
-// Per pass it sweeps the packed panel of K (the partners (K, J) side by side)
-// one group of 16-column slots at a time: the slots' C tiles are loaded into
-// accumulators, the A block's k loop runs with one B fragment per slot and
-// step shared by every 16-row tile of the chunk, and the tiles are written
-// back.  A C element belongs to exactly one unit, and its contributions
-// arrive in the task order (C id, K): no atomics, one fixed order.
+// A unit (one wave) owns a chunk of rows of block row I and walks the sweep
+// list of I: for every A block (I, K), in ascending contracted key K, one or
+// more sweeps over the packed panel of K (the partners (K, J) side by side),
+// eight 16-column slots at most per sweep.  A sweep loads its slots' C tiles
+// into accumulators, runs the A block's k loop with one panel fragment per
+// slot and step shared by every 16-row tile of the chunk, and writes the
+// tiles back.  A C element belongs to exactly one unit, and its
+// contributions arrive in the task order (C id, K): no atomics, one fixed
+// order.
 //
 // MFMA operand layouts (lane l of 64; measured on gfx950):
 //   v_mfma_f64_16x16x4f64   A[l%16][l/16], B[l/16][l%16], D[4v + l/16][l%16]
 //   v_mfma_f32_16x16x4f32   A[l%16][l/16], B[l/16][l%16], D[4(l/16) + v][l%16]
 //   v_mfma_f32_16x16x32bf16 A[l%16][8(l/16)+e], B[8(l/16)+e][l%16], D as f32
+// The kernels compute the transposed tile, D = (panel fragment) x (A
+// fragment)^T: products commute, so every element is the same chain, and
+// lane l then holds rows r0 + 16t + l%16 of four panel columns, so each C
+// load and store instruction covers four columns x 16 consecutive rows.
 //
-//  * sweep_f64<R, NS>: FP64 MFMA.  Each C element accumulates as one fused
-//    multiply-add chain in k order, c = fma(alpha*a, b, c), continued across
-//    passes — bitwise equal to the reference JIT's chain
-//    (src/Kernel/KernelImpl.cpp:594-710) continued over the task list.
-//  * sweep_f32<R, NS>: FP32 MFMA per block product, added into C in FP64.
-//  * sweep_bf3<R, NS>: three-term split-BF16 (x = hi + mid + lo, about 24
-//    significant bits), six bf16 products of order <= 2 per 32-k step, FP32
-//    per block product, FP64 across blocks.
+//  * F = 0, FP64 MFMA: each C element accumulates as one fused multiply-add
+//    chain in k order, c = fma(alpha*a, b, c), continued across sweeps —
+//    bitwise equal to the reference JIT's chain (src/Kernel/KernelImpl.cpp:
+//    594-710) continued over the task list.
+//  * F = 1, FP32 MFMA per block product, added into C in FP64.
+//  * F = 2, three-term split-BF16 (x = hi + mid + lo, about 24 significant
+//    bits), six bf16 products of order <= 2 per 32-k step, FP32 per block
+//    product, FP64 across blocks.
 //  * k_as_exact: the fma chain with one row per lane (bitwise as well).
 
 #include <type_traits>
@@ -36,7 +41,7 @@ typedef short v8s __attribute__((ext_vector_type(8)));
 
 namespace {
 
-// wave-uniform copies (the pass and unit records are the same for every lane)
+// wave-uniform copies (sweep and unit records are the same for every lane)
 __device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ inline int64_t uni(int64_t x)
 {
@@ -45,78 +50,72 @@ __device__ inline int64_t uni(int64_t x)
   return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-struct PassU {
-  int64_t a_off, bt_off;
-  int k, m, ncol, pitch, col0, cof0;
+// Orders this wave's LDS writes before its later LDS reads of other lanes'
+// entries (each wave owns its own LDS slice; no workgroup barrier).
+__device__ inline void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SweepU {
+  int64_t a_off, b_off, c0;
+  int k, m, pitch, ns;
 };
 
-__device__ inline PassU load_pass(const AsPass* passes, int i)
+__device__ inline SweepU to_uni(const AsSweep& v)
 {
-  const AsPass v = passes[i];
-  PassU u;
+  SweepU u;
   u.a_off = uni(v.a_off);
-  u.bt_off = uni(v.bt_off);
+  u.b_off = uni(v.b_off);
+  u.c0 = uni(v.c0);
   u.k = uni(v.k);
   u.m = uni(v.m);
-  u.ncol = uni(v.ncol);
   u.pitch = uni(v.pitch);
-  u.col0 = uni(v.col0);
-  u.cof0 = uni(v.cof0);
+  u.ns = uni(v.ns);
   return u;
 }
 
-// Element offset of C(0, c) for panel column c of the pass, -1 when c is a
-// pad column or its C block is not kept.
-__device__ inline int64_t col_base(const AsArgs& p, const PassU& ps, int c)
+// C offsets of this lane's two sweep columns (lane, lane + 64); -1 past the
+// sweep's 16*ns columns.
+__device__ inline void load_cols(const AsArgs& p, const SweepU& sw, int lane, int64_t& c0, int64_t& c1)
 {
-  if (c >= ps.ncol) return -1;
-  const AsCol cm = p.cols[ps.col0 + c];
-  const int64_t cb = p.cofs[ps.cof0 + cm.p];
-  return cb < 0 ? -1 : cb + static_cast<int64_t>(cm.j) * ps.m;
+  const int n = 16 * sw.ns;
+  const int64_t x0 = p.ccol[sw.c0 + min(lane, n - 1)];
+  const int64_t x1 = p.ccol[sw.c0 + min(lane + 64, n - 1)];
+  c0 = lane < n ? x0 : -1;
+  c1 = lane + 64 < n ? x1 : -1;
 }
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
 // ---------------------------------------------------------------------------
-// FP64 / FP32 MFMA sweeps: R row tiles x NS slots, four k-steps (16 k) per
-// loop iteration (panel rows are zero-padded to a multiple of 16).
-//
 // The A operand of a unit is one stream across all its sweeps: a ring of
 // four register sets holds the A fragments of the next steps, filled three
 // steps ahead, and the last iteration of a sweep already loads the first
-// three steps of the next sweep (same pass, next slot group; or the next
-// pass), so the A stream from HBM does not restart at sweep boundaries.  B
-// is loaded one step ahead into two register sets.  Loads fill raw
-// registers; the k < K mask and alpha are applied by the consumer after the
-// scheduling barrier, so nothing waits for a load before the MFMAs that
-// precede its use.  Every load is in bounds and unconditional: step indexes
+// three steps of the next sweep, so the A stream from HBM does not restart at
+// sweep boundaries.  Every load is in bounds and unconditional: step indexes
 // are clamped, A rows past M read on into the next column (tensor buffers
 // carry kReadSlack bytes of slack), A columns past K re-read column K-1
-// (masked to zero), and panel rows past K are zero.
+// (masked to zero by the consumer), and panel rows past K are zero.  A is read
+// once per unit: non-temporal loads keep the stream from evicting the panels
+// and C tiles that the other units of the block row reuse in L2.
 // ---------------------------------------------------------------------------
 struct AStream {
   const double* a;  // A block (wave-uniform)
   int K, M;
 };
 
-__device__ inline AStream a_stream(const AsArgs& p, const PassU& ps)
-{
-  AStream s;
-  s.a = static_cast<const double*>(p.A) + ps.a_off;
-  s.K = ps.k;
-  s.M = ps.m;
-  return s;
-}
-
-// A fragments of step st of the stream (rows r0 + 16t + li, column 4st + lk)
-// A is read once per unit: non-temporal loads keep the stream from evicting
-// the panels and C tiles that the other units of the block row reuse in L2.
+// A fragments of step st of the stream (rows row + 16t, column 4st + lk).  A
+// blocks of the schedule hold fewer than 2^28 elements, so the element
+// offset fits 32 bits: the load is a uniform base plus a lane offset.
 template <int R>
 __device__ inline void load_a(const AStream& s, int st, int row, int lk, double (&a)[R])
 {
-  const double* col = s.a + static_cast<int64_t>(min(4 * st + lk, s.K - 1)) * s.M + row;
+  const uint32_t off = static_cast<uint32_t>(min(4 * st + lk, s.K - 1) * s.M + row);
 #pragma unroll
-  for (int t = 0; t < R; ++t) a[t] = __builtin_nontemporal_load(col + 16 * t);
+  for (int t = 0; t < R; ++t) a[t] = __builtin_nontemporal_load(s.a + (off + 16u * t));
 }
 
 // Step st of the current sweep's stream when st < nsteps, else step
@@ -136,35 +135,29 @@ __device__ inline void load_a2(const AStream& cu, const AStream& nx, int nsteps,
 template <int F> struct BElem { typedef double type; };
 template <> struct BElem<1> { typedef float type; };
 
-// One sweep.  F = 0: FP64 MFMA, accumulators start from C (each C element is
-// one fma chain in k order); F = 1: FP32 MFMA from zero, then added into C in
-// FP64.  `raw_next` is the record of the pass after ps (loaded by the caller
-// before this sweep); `next_pass` says whether the next sweep belongs to it.
-// Returns the pass of the next sweep.
-//
-// The MFMA computes the transposed tile, D = Panel^T-fragment x A^T-fragment
-// (products commute, so every element is the same fma chain): lane l then
-// holds rows r0 + 16t + l%16 of four panel columns, so each C load and store
-// instruction covers four columns x 16 consecutive rows (128 contiguous bytes
-// per column).  The C offsets of the sweep's columns are staged in LDS.
-template <int R, int NS, int F>
-__device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int r0, int li, int lk,
-                                   double (&ar)[4][R], const AsPass& raw_next, bool next_pass, int64_t* cbs)
+// ---------------------------------------------------------------------------
+// FP64 / FP32 MFMA sweep: R row tiles x NS slots, four k-steps (16 k) per
+// loop iteration (panel rows are zero-padded to a multiple of 16).  A is
+// loaded three steps ahead (ring of four register sets), the panel one step
+// ahead (two sets); in each phase the panel loads (mostly L2 hits) are
+// issued before the A loads (from HBM): the vector memory counter retires in
+// issue order, so waiting for the panel does not wait for the younger A
+// loads.  Loads fill raw registers; the k < K mask and alpha are applied by
+// the consumer after the scheduling barrier.  `cbs` (LDS) holds the C offset
+// of each sweep column.
+// ---------------------------------------------------------------------------
+template <int R, int NS, int F, bool A1>
+__device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const SweepU& nxs, int r0, int li, int lk,
+                                  double (&ar)[4][R], const int64_t* cbs)
 {
   typedef typename BElem<F>::type TB;
   typedef typename std::conditional<F == 0, v4d, v4f>::type Acc;
-  const TB* Bl = static_cast<const TB*>(p.B) + ps.bt_off + 16 * g + li + static_cast<int64_t>(lk) * ps.pitch;
-  const int64_t bstep = 4 * static_cast<int64_t>(ps.pitch);
-  const int K = ps.k;
+  const TB* Bl = static_cast<const TB*>(p.B) + sw.b_off + li + static_cast<int64_t>(lk) * sw.pitch;
+  const int64_t bstep = 4 * static_cast<int64_t>(sw.pitch);
+  const int K = sw.k, M = sw.m;
   double* C = static_cast<double*>(p.C);
-  // column c of the sweep held by this lane for accumulator element v
+  // column of the sweep held by this lane for accumulator element v
   auto colof = [&](int q, int v) { return F == 0 ? 16 * q + 4 * v + lk : 16 * q + 4 * lk + v; };
-  {
-    const int lane = threadIdx.x;
-    if (lane < 16 * NS) cbs[lane] = col_base(p, ps, 16 * g + lane);
-    if (lane + 64 < 16 * NS) cbs[lane + 64] = col_base(p, ps, 16 * g + lane + 64);
-    __syncthreads();
-  }
   const int rl = r0 + li;  // this lane's row in tile 0
   Acc acc[R][NS];
   if constexpr (F == 0) {
@@ -186,37 +179,29 @@ __device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int 
 #pragma unroll
       for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
   }
-  TB br[4][NS];
+  TB b0[NS], b1[NS];
   auto loadB = [&](int st, TB(&b)[NS]) {
     const TB* bs = Bl + st * bstep;
 #pragma unroll
     for (int q = 0; q < NS; ++q) b[q] = bs[16 * q];
   };
   const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
-  loadB(0, br[0]);
-  loadB(1, br[1]);
-  loadB(2, br[2]);
+  loadB(0, b0);
   __builtin_amdgcn_sched_barrier(0);
-  // the next sweep's A stream (its record has arrived by now)
-  PassU nps = ps;
-  if (next_pass) {
-    nps.a_off = uni(raw_next.a_off);
-    nps.bt_off = uni(raw_next.bt_off);
-    nps.k = uni(raw_next.k);
-    nps.m = uni(raw_next.m);
-    nps.ncol = uni(raw_next.ncol);
-    nps.pitch = uni(raw_next.pitch);
-    nps.col0 = uni(raw_next.col0);
-    nps.cof0 = uni(raw_next.cof0);
-  }
-  const AStream cu = a_stream(p, ps), nx = a_stream(p, nps);
+  AStream cu, nx;
+  cu.a = static_cast<const double*>(p.A) + sw.a_off;
+  cu.K = K;
+  cu.M = M;
+  nx.a = static_cast<const double*>(p.A) + nxs.a_off;
+  nx.K = nxs.k;
+  nx.M = nxs.m;
   auto mma = [&](int st, const double(&araw)[R], const TB(&b)[NS]) {
     const bool kin = 4 * st + lk < K;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if constexpr (F == 0) {
         const double y = kin ? araw[t] : 0.0;
-        const double a = p.alpha_one ? y : y * p.alpha;
+        const double a = A1 ? y : y * p.alpha;
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a, acc[t][q], 0, 0, 0);
       } else {
@@ -226,36 +211,29 @@ __device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int 
       }
     }
   };
-  // Both operands are loaded three steps ahead into rings of four register
-  // sets.  B indexes are clamped to the last step; A indexes past it continue
-  // into the next sweep's stream (the A ring then holds its steps 0..2).  In
-  // each phase the B loads (mostly L2 hits) are issued before the A loads
-  // (from HBM): the vector memory counter retires in issue order.
-  const int row = r0 + li;
   for (int st = 0; st < nsteps; st += 4) {
-    loadB(min(st + 3, last), br[3]);
-    load_a2(cu, nx, nsteps, st + 3, row, lk, ar[3]);
+    loadB(st + 1, b1);
+    load_a2(cu, nx, nsteps, st + 3, rl, lk, ar[3]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st, ar[0], br[0]);
+    mma(st, ar[0], b0);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(min(st + 4, last), br[0]);
-    load_a2(cu, nx, nsteps, st + 4, row, lk, ar[0]);
+    loadB(st + 2, b0);
+    load_a2(cu, nx, nsteps, st + 4, rl, lk, ar[0]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st + 1, ar[1], br[1]);
+    mma(st + 1, ar[1], b1);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(min(st + 5, last), br[1]);
-    load_a2(cu, nx, nsteps, st + 5, row, lk, ar[1]);
+    loadB(st + 3, b1);
+    load_a2(cu, nx, nsteps, st + 5, rl, lk, ar[1]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st + 2, ar[2], br[2]);
+    mma(st + 2, ar[2], b0);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(min(st + 6, last), br[2]);
-    load_a2(cu, nx, nsteps, st + 6, row, lk, ar[2]);
+    loadB(min(st + 4, last), b0);
+    load_a2(cu, nx, nsteps, st + 6, rl, lk, ar[2]);
     __builtin_amdgcn_sched_barrier(0);
-    mma(st + 3, ar[3], br[3]);
+    mma(st + 3, ar[3], b1);
     __builtin_amdgcn_sched_barrier(0);
   }
   // write back
-  const int M = ps.m;
   if constexpr (F == 0) {
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -300,101 +278,13 @@ __device__ inline PassU sweep_pipe(const AsArgs& p, const PassU& ps, int g, int 
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  __syncthreads();  // the column offsets are rewritten by the next sweep
-  return nps;
-}
-
-// F: 0 = FP64 MFMA, 1 = FP32 MFMA.  One wave per unit of 16*R rows; sweeps
-// of up to NSMAX slots.
-template <int R, int NSMAX, int F>
-__global__ __launch_bounds__(64) void k_asp(AsArgs p)
-{
-  if (static_cast<int64_t>(blockIdx.x) >= p.nunits) return;
-  const AsUnit u = p.units[blockIdx.x];
-  const int npass = uni(u.npass), pass0 = uni(u.pass0), r0 = uni(u.r0);
-  if (npass == 0) return;
-  const int li = threadIdx.x & 15, lk = threadIdx.x >> 4;
-  const int pend = pass0 + npass;
-  __shared__ int64_t cbs[16 * NSMAX];
-  PassU ps = load_pass(p.passes, pass0);
-  double ar[4][R];
-  {
-    const AStream s0 = a_stream(p, ps);
-    load_a(s0, 0, r0 + li, lk, ar[0]);
-    load_a(s0, 1, r0 + li, lk, ar[1]);
-    load_a(s0, 2, r0 + li, lk, ar[2]);
-  }
-  int pi = pass0, g = 0;
-  while (true) {
-    const int nslots = (ps.ncol + 15) >> 4;
-    const bool more_groups = g + NSMAX < nslots;
-    const bool next_pass = !more_groups && pi + 1 < pend;
-    const AsPass raw_next = p.passes[min(pi + 1, pend - 1)];
-    PassU nps;
-    switch (min(NSMAX, nslots - g)) {
-      case 1: nps = sweep_pipe<R, 1, F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      case 2: nps = sweep_pipe<R, cmin(2, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      case 3: nps = sweep_pipe<R, cmin(3, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      case 4: nps = sweep_pipe<R, cmin(4, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      case 5: nps = sweep_pipe<R, cmin(5, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      case 6: nps = sweep_pipe<R, cmin(6, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      case 7: nps = sweep_pipe<R, cmin(7, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-      default: nps = sweep_pipe<R, cmin(8, NSMAX), F>(p, ps, g, r0, li, lk, ar, raw_next, next_pass, cbs); break;
-    }
-    if (more_groups) {
-      g += NSMAX;
-    } else if (next_pass) {
-      ps = nps;
-      ++pi;
-      g = 0;
-    } else {
-      break;
-    }
-  }
-}
-
-// C += alpha * (FP32 block product) in FP64 for the split-BF16 sweep, 8/R
-// slots at a time (32 doubles per lane in flight).  The f32 accumulator
-// layout puts rows 4*(l/16) + v of column l%16 in lane l.
-template <int R, int NS>
-__device__ inline void as_lowp_epilogue(const AsArgs& p, const PassU& ps, int g, int r0, int lk, int li,
-                                        const v4f (&acc)[R][NS])
-{
-  constexpr int G = NS < 8 / R ? NS : 8 / R;
-  double* C = static_cast<double*>(p.C);
-  const int M = ps.m;
-#pragma unroll
-  for (int q0 = 0; q0 < NS; q0 += G) {
-    int64_t cb[G];
-    double cv[G][R][4];
-#pragma unroll
-    for (int x = 0; x < G; ++x) {
-      cb[x] = q0 + x < NS ? col_base(p, ps, 16 * (g + q0 + x) + li) : -1;
-      const double* cq = C + (cb[x] >= 0 ? cb[x] : 0) + r0 + 4 * lk;
-#pragma unroll
-      for (int t = 0; t < R; ++t)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) cv[x][t][v] = cq[16 * t + v];
-    }
-#pragma unroll
-    for (int x = 0; x < G; ++x) {
-      if (q0 + x >= NS) break;
-      double* cq = C + (cb[x] >= 0 ? cb[x] : 0) + r0 + 4 * lk;
-#pragma unroll
-      for (int t = 0; t < R; ++t)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (cb[x] >= 0 && r0 + 16 * t + 4 * lk + v < M)
-            cq[16 * t + v] = cv[x][t][v] + p.alpha * static_cast<double>(acc[t][q0 + x][v]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
 }
 
 // ---------------------------------------------------------------------------
 // Three-term split-BF16 sweep.  x (rounded to FP32) = hi + mid + lo, each a
 // bf16 (round to nearest even); per 32-k step six products of order <= 2,
-// smallest first: lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi.
+// smallest first: lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi.  The raw A
+// values of the next step are loaded while the current step's products run.
 // ---------------------------------------------------------------------------
 __device__ inline unsigned short bf16_bits(float x)
 {
@@ -413,39 +303,37 @@ __device__ inline void split3(double xd, unsigned short& h, unsigned short& m, u
 }
 
 template <int R, int NS>
-__device__ inline void sweep_bf3(const AsArgs& p, const PassU& ps, int g, int r0, int nt, int li, int lk)
+__device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int li, int lk, const int64_t* cbs)
 {
-  const double* A = static_cast<const double*>(p.A) + ps.a_off;
-  const int kp = ps.pitch;
-  const unsigned short* Bl = static_cast<const unsigned short*>(p.B) + ps.bt_off +
-                             static_cast<int64_t>(16 * g + li) * kp + 8 * lk;
+  const double* A = static_cast<const double*>(p.A) + sw.a_off;
+  const int kp = sw.pitch;
+  const unsigned short* Bl =
+      static_cast<const unsigned short*>(p.B) + sw.b_off + static_cast<int64_t>(li) * kp + 8 * lk;
   const int64_t plane = p.bt_total;
   const int64_t qstep = 16 * static_cast<int64_t>(kp);
   double* C = static_cast<double*>(p.C);
-  const int K = ps.k, M = ps.m;
+  const int K = sw.k, M = sw.m;
+  const int rl = r0 + li;
   v4f acc[R][NS];
 #pragma unroll
   for (int t = 0; t < R; ++t)
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
-  int arow[R];
-#pragma unroll
-  for (int t = 0; t < R; ++t) arow[t] = min(r0 + 16 * t + li, M - 1);
   const int nsteps = (K + kBfK - 1) / kBfK;
   double x_nx[R][8];
-  auto load_a = [&](int st) {
+  auto load_a8 = [&](int st) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int kk = kBfK * st + 8 * lk + e;
-      const double* col = A + static_cast<int64_t>(min(kk, K - 1)) * M;
+      const uint32_t off = static_cast<uint32_t>(min(kk, K - 1) * M + rl);
 #pragma unroll
       for (int t = 0; t < R; ++t) {
-        const double x = col[arow[t]];
+        const double x = A[off + 16u * t];
         x_nx[t][e] = kk < K ? x : 0.0;
       }
     }
   };
-  load_a(0);
+  load_a8(0);
   for (int st = 0; st < nsteps; ++st) {
     v8s ah[R], am[R], al[R];
 #pragma unroll
@@ -458,7 +346,7 @@ __device__ inline void sweep_bf3(const AsArgs& p, const PassU& ps, int g, int r0
         am[t][e] = static_cast<short>(m);
         al[t][e] = static_cast<short>(l);
       }
-    load_a(min(st + 1, nsteps - 1));
+    load_a8(min(st + 1, nsteps - 1));
     const int k0 = kBfK * st;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -468,56 +356,106 @@ __device__ inline void sweep_bf3(const AsArgs& p, const PassU& ps, int g, int r0
       const v8s bl = *reinterpret_cast<const v8s*>(b + 2 * plane);
 #pragma unroll
       for (int t = 0; t < R; ++t) {
-        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], bh, acc[t][q], 0, 0, 0);
-        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[t], bm, acc[t][q], 0, 0, 0);
-        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bl, acc[t][q], 0, 0, 0);
-        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[t], bh, acc[t][q], 0, 0, 0);
-        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bm, acc[t][q], 0, 0, 0);
-        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bh, acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm, am[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, am[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm, ah[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah[t], acc[t][q], 0, 0, 0);
       }
     }
   }
-  as_lowp_epilogue<R, NS>(p, ps, g, r0, lk, li, acc);
-}
-
-template <int R, int F, int NS>
-__device__ inline void run_sweep(const AsArgs& p, const PassU& ps, int g, int r0, int nt, int li, int lk)
-{
-  static_assert(F == 2, "the MFMA forms run k_asp");
-  sweep_bf3<R, NS>(p, ps, g, r0, nt, li, lk);
-}
-
-// Split-BF16: one wave per unit of 16*R rows; sweeps of up to NSMAX slots.
-template <int R, int NSMAX, int F>
-__global__ __launch_bounds__(64) void k_as(AsArgs p)
-{
-  if (static_cast<int64_t>(blockIdx.x) >= p.nunits) return;
-  const AsUnit u = p.units[blockIdx.x];
-  const int npass = uni(u.npass), pass0 = uni(u.pass0), r0 = uni(u.r0);
-  const int li = threadIdx.x & 15, lk = threadIdx.x >> 4;
-  for (int pi = pass0; pi < pass0 + npass; ++pi) {
-    const PassU ps = load_pass(p.passes, pi);
-    const int nt = min(R, (ps.m - r0 + 15) >> 4);
-    const int nslots = (ps.ncol + 15) >> 4;
-    for (int g = 0; g < nslots; g += NSMAX) {
-      switch (min(NSMAX, nslots - g)) {
-        case 1: run_sweep<R, F, 1>(p, ps, g, r0, nt, li, lk); break;
-        case 2: run_sweep<R, F, cmin(2, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-        case 3: run_sweep<R, F, cmin(3, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-        case 4: run_sweep<R, F, cmin(4, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-        case 5: run_sweep<R, F, cmin(5, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-        case 6: run_sweep<R, F, cmin(6, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-        case 7: run_sweep<R, F, cmin(7, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-        default: run_sweep<R, F, cmin(8, NSMAX)>(p, ps, g, r0, nt, li, lk); break;
-      }
+  // C += alpha * (FP32 block product) in FP64; lane l holds rows r0 + 16t +
+  // l%16 of columns 16q + 4(l/16) + v.
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    double cv[4][R];
+    int64_t cb[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      cb[v] = cbs[16 * q + 4 * lk + v];
+      const double* cq = C + (cb[v] >= 0 ? cb[v] : 0) + rl;
+#pragma unroll
+      for (int t = 0; t < R; ++t) cv[v][t] = cq[16 * t];
     }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double* cq = C + (cb[v] >= 0 ? cb[v] : 0) + rl;
+#pragma unroll
+      for (int t = 0; t < R; ++t)
+        if (cb[v] >= 0 && rl + 16 * t < M) cq[16 * t] = cv[v][t] + p.alpha * static_cast<double>(acc[t][q][v]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int R, int NS, int F, bool A1>
+__device__ inline void run_sweep(const AsArgs& p, const SweepU& sw, const SweepU& nx, int r0, int li, int lk,
+                                 double (&ar)[4][R], const int64_t* cbs)
+{
+  if constexpr (F == 2) sweep_bf3<R, NS>(p, sw, r0, li, lk, cbs);
+  else sweep_mfma<R, NS, F, A1>(p, sw, nx, r0, li, lk, ar, cbs);
+}
+
+// One wave per unit of 16*R rows, WPG waves (units of one block row) per
+// workgroup; F: 0 = FP64 MFMA, 1 = FP32 MFMA, 2 = split-BF16; A1: alpha == 1.
+// Waves are independent (no workgroup barrier): an empty unit just returns.
+// The records of the next two sweeps and the next sweep's column offsets are
+// loaded one sweep ahead, so a sweep starts without a dependent load chain.
+template <int R, int NSMAX, int F, int WPG, bool A1>
+__global__ __launch_bounds__(64 * WPG) void k_asp(AsArgs p)
+{
+  static_assert(16 * NSMAX <= 128, "two column offsets per lane");
+  const int wave = WPG == 1 ? 0 : uni(static_cast<int>(threadIdx.x >> 6));
+  const int64_t ui = static_cast<int64_t>(blockIdx.x) * WPG + wave;
+  if (ui >= p.nunits) return;
+  const AsUnit u = p.units[ui];
+  const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
+  if (nsw == 0) return;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  __shared__ int64_t cbs_all[WPG][128];
+  int64_t* cbs = cbs_all[wave];
+  SweepU cur = to_uni(p.sweeps[s0]);
+  SweepU nxt = to_uni(p.sweeps[s0 + min(1, nsw - 1)]);
+  AsSweep raw2 = p.sweeps[s0 + min(2, nsw - 1)];
+  int64_t cc0, cc1;
+  load_cols(p, cur, lane, cc0, cc1);
+  double ar[4][R];
+  if constexpr (F != 2) {
+    AStream s;
+    s.a = static_cast<const double*>(p.A) + cur.a_off;
+    s.K = cur.k;
+    s.M = cur.m;
+    load_a(s, 0, r0 + li, lk, ar[0]);
+    load_a(s, 1, r0 + li, lk, ar[1]);
+    load_a(s, 2, r0 + li, lk, ar[2]);
+  }
+  for (int j = 0; j < nsw; ++j) {
+    cbs[lane] = cc0;
+    cbs[lane + 64] = cc1;
+    wave_sync();
+    load_cols(p, nxt, lane, cc0, cc1);
+    switch (min(cur.ns, NSMAX)) {
+      case 1: run_sweep<R, 1, F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 2: run_sweep<R, cmin(2, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 3: run_sweep<R, cmin(3, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 4: run_sweep<R, cmin(4, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 5: run_sweep<R, cmin(5, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 6: run_sweep<R, cmin(6, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 7: run_sweep<R, cmin(7, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      default: run_sweep<R, cmin(8, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+    }
+    wave_sync();  // the column offsets are rewritten by the next sweep
+    cur = nxt;
+    nxt = to_uni(raw2);
+    raw2 = p.sweeps[s0 + min(j + 3, nsw - 1)];
   }
 }
 
 // ---------------------------------------------------------------------------
 // A-stationary fma chain (bitwise equal to the oracle).  One wave per 64-row
-// unit, lane = one C row; per sweep up to kAsxSlots slots of kAsxCols panel
-// columns keep their C values in registers while k runs ascending:
+// unit, lane = one C row; per chunk of 64 sweep columns (kAsxSlots slots of
+// kAsxCols) the C values stay in registers while k runs ascending:
 // c = fma(alpha*A(i,k), B(k,j), c).  B(k, c0..c0+7) is wave-uniform and
 // contiguous in the panel.
 // ---------------------------------------------------------------------------
@@ -526,45 +464,46 @@ __global__ __launch_bounds__(64) void k_as_exact(AsArgs p)
   constexpr int S = kAsxSlots, W = kAsxCols;
   if (static_cast<int64_t>(blockIdx.x) >= p.nunits) return;
   const AsUnit u = p.units[blockIdx.x];
-  const int npass = uni(u.npass), pass0 = uni(u.pass0), r0 = uni(u.r0);
+  const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
   const double* A = static_cast<const double*>(p.A);
   const double* BT = static_cast<const double*>(p.B);
   double* C = static_cast<double*>(p.C);
   const int row = r0 + static_cast<int>(threadIdx.x);
-  for (int pi = pass0; pi < pass0 + npass; ++pi) {
-    const PassU ps = load_pass(p.passes, pi);
-    const int K = ps.k, M = ps.m;
+  for (int j = 0; j < nsw; ++j) {
+    const SweepU sw = to_uni(p.sweeps[s0 + j]);
+    const int K = sw.k, M = sw.m;
     const bool rowok = row < M;
-    const double* Ab = A + ps.a_off + min(row, M - 1);
-    for (int c0 = 0; c0 < ps.ncol; c0 += S * W) {
-      const int ns = min(S, (ps.ncol - c0 + W - 1) / W);
+    const double* Ab = A + sw.a_off + min(row, M - 1);
+    const int ncol = 16 * sw.ns;
+    for (int c0 = 0; c0 < ncol; c0 += S * W) {
+      const int ns = min(S, (ncol - c0) / W);
       double acc[S][W];
 #pragma unroll
       for (int q = 0; q < S; ++q)
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const int64_t cb = q < ns ? col_base(p, ps, c0 + q * W + j) : -1;
-          acc[q][j] = (cb >= 0 && rowok) ? C[cb + row] : 0.0;
+        for (int x = 0; x < W; ++x) {
+          const int64_t cb = q < ns ? p.ccol[sw.c0 + c0 + q * W + x] : -1;
+          acc[q][x] = (cb >= 0 && rowok) ? C[cb + row] : 0.0;
         }
-      const double* bt = BT + ps.bt_off + c0;
+      const double* bt = BT + sw.b_off + c0;
       for (int k = 0; k < K; ++k) {
         double a = Ab[static_cast<int64_t>(k) * M];
         if (!p.alpha_one) a = a * p.alpha;
-        const double* bk = bt + static_cast<int64_t>(k) * ps.pitch;
+        const double* bk = bt + static_cast<int64_t>(k) * sw.pitch;
 #pragma unroll
         for (int q = 0; q < S; ++q) {
           if (q < ns) {
 #pragma unroll
-            for (int j = 0; j < W; ++j) acc[q][j] = __fma_rn(a, bk[q * W + j], acc[q][j]);
+            for (int x = 0; x < W; ++x) acc[q][x] = __fma_rn(a, bk[q * W + x], acc[q][x]);
           }
         }
       }
 #pragma unroll
       for (int q = 0; q < S; ++q)
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const int64_t cb = q < ns ? col_base(p, ps, c0 + q * W + j) : -1;
-          if (cb >= 0 && rowok) C[cb + row] = acc[q][j];
+        for (int x = 0; x < W; ++x) {
+          const int64_t cb = q < ns ? p.ccol[sw.c0 + c0 + q * W + x] : -1;
+          if (cb >= 0 && rowok) C[cb + row] = acc[q][x];
         }
     }
   }
@@ -619,28 +558,38 @@ __global__ __launch_bounds__(256) void k_scale(double* c, int64_t n, double beta
 
 }  // namespace
 
-int as_unit_rows(AsClass k)
+int as_group_waves(AsClass k)
 {
   switch (k) {
-    case AsClass::Exact:
-    case AsClass::Bf16x3W: return kAsUnitRowsW;
-    default: return kAsUnitRows;
+    case AsClass::Mfma64G:
+    case AsClass::Mfma32G:
+    case AsClass::Bf16x3G: return kAsGroupWaves;
+    default: return 1;
   }
 }
 
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
 {
   if (a.nunits <= 0) return;
-  const dim3 g(static_cast<unsigned>(a.nunits)), b(64);
-  static_assert(kAsUnitRows == 32 && kAsUnitRowsW == 64, "unit rows: 2 and 4 MFMA tiles");
+  constexpr int G = kAsGroupWaves;
+  const dim3 g1(static_cast<unsigned>(a.nunits)), b1(64);
+  const dim3 gg(static_cast<unsigned>((a.nunits + G - 1) / G)), bg(64 * G);
+  static_assert(kAsUnitRows == 32 && kAsUnitRowsW == 64, "unit rows: two MFMA tiles; 64 lanes");
+  static_assert(kAsSweepSlots == 8, "sweep slots");
   switch (k) {
-    case AsClass::Mfma64: hipLaunchKernelGGL((k_asp<2, 8, 0>), g, b, 0, s, a); break;
-    case AsClass::Mfma64W: hipLaunchKernelGGL((k_asp<2, 4, 0>), g, b, 0, s, a); break;
-    case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1>), g, b, 0, s, a); break;
-    case AsClass::Mfma32W: hipLaunchKernelGGL((k_asp<2, 4, 1>), g, b, 0, s, a); break;
-    case AsClass::Bf16x3: hipLaunchKernelGGL((k_as<2, 4, 2>), g, b, 0, s, a); break;
-    case AsClass::Bf16x3W: hipLaunchKernelGGL((k_as<4, 4, 2>), g, b, 0, s, a); break;
-    case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g, b, 0, s, a); break;
+    case AsClass::Mfma64:
+      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, 8, 0, 1, true>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<2, 8, 0, 1, false>), g1, b1, 0, s, a);
+      break;
+    case AsClass::Mfma64G:
+      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, 8, 0, G, true>), gg, bg, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<2, 8, 0, G, false>), gg, bg, 0, s, a);
+      break;
+    case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false>), g1, b1, 0, s, a); break;
+    case AsClass::Mfma32G: hipLaunchKernelGGL((k_asp<2, 8, 1, G, false>), gg, bg, 0, s, a); break;
+    case AsClass::Bf16x3: hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false>), g1, b1, 0, s, a); break;
+    case AsClass::Bf16x3G: hipLaunchKernelGGL((k_asp<2, 8, 2, G, false>), gg, bg, 0, s, a); break;
+    case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g1, b1, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }
   SHT_HIP(hipGetLastError());

@@ -564,6 +564,9 @@ void build_as_schedule(sht_plan* P)
     if (bK[x] != bK[y]) return bK[x] < bK[y];
     return bJ[x] < bJ[y];
   });
+  struct AsCol {  // panel column -> (partner index within the panel, column of the partner)
+    int32_t p, j;
+  };
   struct Panel {
     int64_t bt_off;
     int32_t k, ncol, pitch, col0, npart;
@@ -616,52 +619,66 @@ void build_as_schedule(sht_plan* P)
     panels.push_back(pn);
     u = e;
   }
-  // passes and per-pass partner C offsets; block rows with their cost
-  std::vector<AsPass> passes;
-  std::vector<int64_t> cofs;
+  // sweep list per block row, with the C offset of every sweep column
+  std::vector<AsSweep> sweeps;
+  std::vector<int64_t> ccol;
+  std::vector<int64_t> pcof;  // partner C offsets of the current A block
   struct Row {
-    int32_t pass0, npass;
+    int32_t sweep0, nsweep;
     int64_t rows;
-    double work;  // sum over passes of k * panel columns
+    double work;  // sum over sweeps of k * columns
   };
   std::vector<Row> brows;
   int64_t t = 0;
   while (t < nt) {
     const int64_t I = aI[ta[order[t]]];
-    Row row{static_cast<int32_t>(passes.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
+    Row row{static_cast<int32_t>(sweeps.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
     while (t < nt && aI[ta[order[t]]] == I) {
       const int64_t a = ta[order[t]];
       const Panel& pn = panels[panel_of[tb[order[t]]]];
-      AsPass ps{};
-      ps.a_off = A->offsets[a];
-      ps.bt_off = pn.bt_off;
-      ps.k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
-      ps.m = static_cast<int32_t>(row.rows);
-      ps.ncol = pn.ncol;
-      ps.pitch = pn.pitch;
-      ps.col0 = pn.col0;
-      ps.cof0 = static_cast<int32_t>(cofs.size());
-      SHT_REQUIRE(cofs.size() + pn.npart < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
-      cofs.resize(cofs.size() + pn.npart, -1);
+      pcof.assign(pn.npart, -1);
       while (t < nt && ta[order[t]] == a) {
-        cofs[ps.cof0 + part_of[tb[order[t]]]] = C->offsets[tc[order[t]]];
+        pcof[part_of[tb[order[t]]]] = C->offsets[tc[order[t]]];
         ++t;
       }
-      row.work += static_cast<double>(ps.k) * ((pn.ncol + kBtPad - 1) / kBtPad * kBtPad);
-      passes.push_back(ps);
+      const int32_t k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
+      const int nslots = (pn.ncol + kBtPad - 1) / kBtPad;
+      for (int g = 0; g < nslots; g += kAsSweepSlots) {
+        AsSweep sw{};
+        sw.a_off = A->offsets[a];
+        sw.b_off = bf3 ? pn.bt_off + static_cast<int64_t>(16 * g) * pn.pitch : pn.bt_off + 16 * g;
+        sw.c0 = static_cast<int64_t>(ccol.size());
+        sw.k = k;
+        sw.m = static_cast<int32_t>(row.rows);
+        sw.pitch = pn.pitch;
+        sw.ns = std::min(kAsSweepSlots, nslots - g);
+        for (int c = 16 * g; c < 16 * (g + sw.ns); ++c) {
+          int64_t v = -1;
+          if (c < pn.ncol) {
+            const AsCol& cm = cols[pn.col0 + c];
+            if (pcof[cm.p] >= 0) v = pcof[cm.p] + static_cast<int64_t>(cm.j) * row.rows;
+          }
+          ccol.push_back(v);
+        }
+        row.work += static_cast<double>(k) * 16 * sw.ns;
+        sweeps.push_back(sw);
+      }
     }
-    row.npass = static_cast<int32_t>(passes.size()) - row.pass0;
+    row.nsweep = static_cast<int32_t>(sweeps.size()) - row.sweep0;
     brows.push_back(row);
   }
-  SHT_REQUIRE(passes.size() < (size_t(1) << 31) && cols.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED,
-              "schedule too large");
+  SHT_REQUIRE(sweeps.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
   // XCD-aware order.  Workgroup g runs on XCD g % 8 (round-robin dispatch),
   // and every XCD has its own L2: all units of a block row go to one XCD, so
   // the row's panels are shared in that L2.  Rows are dealt to the XCD with
   // the least accumulated work (longest first), then the eight queues are
   // interleaved; short queues are padded with empty units.
   constexpr int kXcd = 8;
-  auto build = [&](int step) {
+  // Units of one block row are grouped per_wg to a workgroup (padded with
+  // empty units, so a workgroup never mixes rows): the waves of a workgroup
+  // run the same pass sequence on neighbouring row chunks and share the
+  // panel fragments through the CU's L1.
+  auto build = [&](int step, int per_wg) {
     std::vector<int64_t> ro(brows.size());
     std::iota(ro.begin(), ro.end(), 0);
     std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) {
@@ -673,29 +690,32 @@ void build_as_schedule(sht_plan* P)
       const Row& br = brows[r];
       const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
       for (int64_t r0 = 0; r0 < br.rows; r0 += step)
-        q[x].push_back(AsUnit{br.pass0, br.npass, static_cast<int32_t>(r0), 0});
+        q[x].push_back(AsUnit{br.sweep0, br.nsweep, static_cast<int32_t>(r0), 0});
+      while (q[x].size() % per_wg) q[x].push_back(AsUnit{0, 0, 0, 0});
       load[x] += br.work * static_cast<double>(br.rows);
     }
     size_t longest = 0;
     for (const auto& v : q) longest = std::max(longest, v.size());
     std::vector<AsUnit> out;
     out.reserve(longest * kXcd);
-    for (size_t slot = 0; slot < longest; ++slot)
-      for (int x = 0; x < kXcd; ++x) out.push_back(slot < q[x].size() ? q[x][slot] : AsUnit{0, 0, 0, 0});
-    while (!out.empty() && out.back().npass == 0) out.pop_back();
+    for (size_t slot = 0; slot < longest; slot += per_wg)
+      for (int x = 0; x < kXcd; ++x)
+        for (int w = 0; w < per_wg; ++w)
+          out.push_back(slot + w < q[x].size() ? q[x][slot + w] : AsUnit{0, 0, 0, 0});
+    while (!out.empty() && out.back().nsweep == 0) out.pop_back();
+    while (out.size() % per_wg) out.push_back(AsUnit{0, 0, 0, 0});
     return out;
   };
-  std::vector<AsUnit> us = exact ? std::vector<AsUnit>() : build(kAsUnitRows);
-  std::vector<AsUnit> usw = build(kAsUnitRowsW);
+  std::vector<AsUnit> us = build(exact ? kAsUnitRowsW : kAsUnitRows, 1);
+  std::vector<AsUnit> usw = exact ? std::vector<AsUnit>() : build(kAsUnitRows, kAsGroupWaves);
   Arena& arena = P->ctx->arena;
   auto upload = [&](const void* src, size_t bytes) {
     void* d = arena.allocate(std::max<size_t>(bytes, 1));
     if (bytes) SHT_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
     return d;
   };
-  P->d_as_pass = static_cast<AsPass*>(upload(passes.data(), passes.size() * sizeof(AsPass)));
-  P->d_as_col = static_cast<AsCol*>(upload(cols.data(), cols.size() * sizeof(AsCol)));
-  P->d_as_cof = static_cast<int64_t*>(upload(cofs.data(), cofs.size() * sizeof(int64_t)));
+  P->d_as_sweep = static_cast<AsSweep*>(upload(sweeps.data(), sweeps.size() * sizeof(AsSweep)));
+  P->d_as_ccol = static_cast<int64_t*>(upload(ccol.data(), ccol.size() * sizeof(int64_t)));
   P->d_as_unit = static_cast<AsUnit*>(upload(us.data(), us.size() * sizeof(AsUnit)));
   P->d_as_unitw = static_cast<AsUnit*>(upload(usw.data(), usw.size() * sizeof(AsUnit)));
   P->d_bt_rec = static_cast<BtRec*>(upload(bt.data(), bt.size() * sizeof(BtRec)));
@@ -709,12 +729,24 @@ void build_as_schedule(sht_plan* P)
   P->bt_nrec = static_cast<int64_t>(bt.size());
   P->as_nunits = static_cast<int64_t>(us.size());
   P->as_nunitsw = static_cast<int64_t>(usw.size());
-  P->as_npass = static_cast<int64_t>(passes.size());
+  P->as_nsweep = static_cast<int64_t>(sweeps.size());
   P->as_npanel = static_cast<int64_t>(panels.size());
   P->as_built = true;
 }
 
-bool as_supported(const sht_plan*) { return true; }
+// The A-stationary kernels address an A block with 32-bit element offsets:
+// every A block must hold fewer than 2^28 elements (bound by the largest
+// block size of each mode).
+bool as_supported(const sht_plan* P)
+{
+  double vmax = 1.0;
+  for (int ax = 0; ax < P->A->N; ++ax) {
+    int b = 0;
+    for (int x : P->A->bs[ax]) b = std::max(b, x);
+    vmax *= b;
+  }
+  return vmax < static_cast<double>(1 << 28);
+}
 
 BtFormat bt_format(const sht_plan* P)
 {
@@ -726,9 +758,9 @@ AsClass as_class(const sht_plan* P)
   const bool wide = P->schedule == 2;
   switch (P->prec) {
     case SHT_PREC_FP64_EXACT: return AsClass::Exact;
-    case SHT_PREC_FP32: return wide ? AsClass::Mfma32W : AsClass::Mfma32;
-    case SHT_PREC_BF16X3: return wide ? AsClass::Bf16x3W : AsClass::Bf16x3;
-    default: return wide ? AsClass::Mfma64W : AsClass::Mfma64;
+    case SHT_PREC_FP32: return wide ? AsClass::Mfma32G : AsClass::Mfma32;
+    case SHT_PREC_BF16X3: return wide ? AsClass::Bf16x3G : AsClass::Bf16x3;
+    default: return wide ? AsClass::Mfma64G : AsClass::Mfma64;
   }
 }
 
@@ -804,11 +836,10 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.B = P->d_bt;
       g.bt_total = P->bt_total;
       g.C = Cd;
-      g.passes = P->d_as_pass;
-      g.cols = P->d_as_col;
-      g.cofs = P->d_as_cof;
+      g.sweeps = P->d_as_sweep;
+      g.ccol = P->d_as_ccol;
       const AsClass cls = as_class(P);
-      const bool wide = as_unit_rows(cls) == kAsUnitRowsW;
+      const bool wide = as_group_waves(cls) > 1;
       g.units = wide ? P->d_as_unitw : P->d_as_unit;
       g.nunits = wide ? P->as_nunitsw : P->as_nunits;
       g.alpha = alpha;
@@ -933,8 +964,8 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 
 const char* sht_kclass_name(int k)
 {
-  static const char* as_names[] = {"as_mfma64", "as_exact", "as_mfma64_w", "as_mfma32",
-                                   "as_bf16x3", "as_mfma32_w", "as_bf16x3_w"};
+  static const char* as_names[] = {"as_mfma64", "as_exact", "as_mfma64_g", "as_mfma32",
+                                   "as_bf16x3", "as_mfma32_g", "as_bf16x3_g"};
   if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};

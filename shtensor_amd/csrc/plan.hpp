@@ -62,25 +62,21 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
 // A panel = the B blocks of one contracted key K (the partners of every A
 // block (I, K)) in ascending block column J, converted once per execution
 // into one packed matrix whose columns are the partners' columns side by
-// side.  A pass = one A block (I, K) against the panel of K, swept 16 panel
-// columns (one slot) at a time.  A unit = one chunk of block row I (32 or 64
-// rows) walking all passes of I in ascending K, so each A element is read once
-// per unit and every C element is updated in place in the task order
+// side.  The sweep list of block row I: for every A block (I, K), in
+// ascending K, the panel of K cut into sweeps of at most kAsSweepSlots
+// 16-column slots.  A unit = one chunk of block row I (32 or 64 rows)
+// walking the sweep list of I, so each A element is read once per sweep of
+// its block and every C element is updated in place in the task order
 // (C id, K) of the task list.
-struct AsPass {
+constexpr int kAsSweepSlots = 8;
+struct AsSweep {
   int64_t a_off;   // A block (m x k, column-major)
-  int64_t bt_off;  // panel, in elements of the B format
+  int64_t b_off;   // first sweep column in the panel, in elements of the B format
+  int64_t c0;      // first entry of the sweep's 16*ns column offsets (AsArgs::ccol)
   int32_t k;       // contracted length of the A block
   int32_t m;       // rows of block row I (= rows of every C block (I, J))
-  int32_t ncol;    // packed columns of the panel
-  int32_t pitch;   // F64T/F32T: row pitch (ncol rounded up to 16); BF16X3: contracted pad (k rounded up to 32)
-  int32_t col0;    // first entry of the panel's column map
-  int32_t cof0;    // first entry of this pass's partner C offsets
-};
-// Panel column -> (partner index within the panel, column of the partner).
-struct AsCol {
-  int32_t p;
-  int32_t j;
+  int32_t pitch;   // F64T/F32T: panel row pitch (columns rounded up to 16); BF16X3: contracted pad (k rounded up to 32)
+  int32_t ns;      // 16-column slots of the sweep (1..kAsSweepSlots)
 };
 constexpr int kBtPad = 16;  // panel columns are padded (zeros) to a multiple of 16
 constexpr int kBfK = 32;    // k granularity of the bf16 MFMA (16x16x32)
@@ -103,9 +99,9 @@ enum class BtFormat : int { F64T = 0, F32T = 1, BF16X3 = 2 };
 void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const double* B, void* BT, int64_t bt_total,
                hipStream_t s);
 struct AsUnit {
-  int32_t pass0;
-  int32_t npass;
-  int32_t r0;    // first row of the chunk
+  int32_t sweep0;  // first sweep of the block row
+  int32_t nsweep;  // sweeps of the block row (0: empty unit)
+  int32_t r0;      // first row of the chunk
   int32_t pad;
 };
 struct AsArgs {
@@ -113,26 +109,27 @@ struct AsArgs {
   const void* B;      // panels (BtFormat of the plan)
   void* C;
   int64_t bt_total;   // elements per bf16 plane
-  const AsPass* passes;
-  const AsCol* cols;
-  const int64_t* cofs;  // per pass and partner: C block offset, -1 if the C block is not kept
+  const AsSweep* sweeps;
+  const int64_t* ccol;  // per sweep column: element offset of C(0, column), -1 for pad columns / dropped blocks
   const AsUnit* units;
   int64_t nunits;
   double alpha;
   int alpha_one;
 };
-// Kernel classes.  Mfma64/Mfma32/Bf16x3 run 32-row units (two 16-row MFMA
-// tiles per wave), the *W variants 64-row units (four tiles); Exact runs
-// 64-row units with one row per lane.
-enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64W = 2, Mfma32 = 3, Bf16x3 = 4, Mfma32W = 5, Bf16x3W = 6,
+// Kernel classes.  The MFMA classes run 32-row units (two 16-row MFMA tiles
+// per wave): Mfma64/Mfma32/Bf16x3 one wave per workgroup, the *G variants
+// kAsGroupWaves waves (units of one block row) per workgroup sharing panel
+// fragments through L1.  Exact runs 64-row units with one row per lane.
+enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64G = 2, Mfma32 = 3, Bf16x3 = 4, Mfma32G = 5, Bf16x3G = 6,
                            Count };
-constexpr int kAsUnitRows = 32;    // rows of a unit of the 2-tile kernels
-constexpr int kAsUnitRowsW = 64;   // rows of a unit of the 4-tile and fma-chain kernels
+constexpr int kAsUnitRows = 32;    // rows of a unit of the MFMA kernels
+constexpr int kAsUnitRowsW = 64;   // rows of a unit of the fma-chain kernel
+constexpr int kAsGroupWaves = 4;   // waves per workgroup of the *G classes
 constexpr int kAsxCols = 8;        // columns per fma-chain slot
 constexpr int kAsxSlots = 8;       // slots per fma-chain sweep
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
-// rows of the units a class runs (kAsUnitRows or kAsUnitRowsW)
-int as_unit_rows(AsClass k);
+// waves per workgroup of a class (its units come grouped that way)
+int as_group_waves(AsClass k);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
 void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);
 
@@ -184,17 +181,16 @@ struct sht_plan {
   double bytes_in = 0.0;  // A + B + C write
   double bytes_c = 0.0;   // C read (beta != 0)
   // A-stationary schedule (built on demand)
-  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary 32-row units, 2: A-stationary 64-row units
+  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary one-wave units, 2: A-stationary grouped units
   bool as_built = false;
-  sht::AsPass* d_as_pass = nullptr;
-  sht::AsCol* d_as_col = nullptr;
-  int64_t* d_as_cof = nullptr;
-  sht::AsUnit* d_as_unit = nullptr;   // 32-row units
-  sht::AsUnit* d_as_unitw = nullptr;  // 64-row units
+  sht::AsSweep* d_as_sweep = nullptr;
+  int64_t* d_as_ccol = nullptr;
+  sht::AsUnit* d_as_unit = nullptr;   // one unit per workgroup (32 rows; 64 for the fma chain)
+  sht::AsUnit* d_as_unitw = nullptr;  // kAsGroupWaves units of one block row per workgroup
   sht::BtRec* d_bt_rec = nullptr;
   void* d_bt = nullptr;
   int64_t bt_nrec = 0, bt_total = 0;
-  int64_t as_nunits = 0, as_nunitsw = 0, as_npass = 0, as_npanel = 0;
+  int64_t as_nunits = 0, as_nunitsw = 0, as_nsweep = 0, as_npanel = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   double graph_alpha = 0.0, graph_beta = 0.0;
@@ -204,8 +200,8 @@ struct sht_plan {
     sht::Arena& a = ctx->arena;
     for (void* p : {static_cast<void*>(d_ta), static_cast<void*>(d_tb), static_cast<void*>(d_tc),
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
-                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_pass),
-                    static_cast<void*>(d_as_col), static_cast<void*>(d_as_cof), static_cast<void*>(d_as_unit),
+                    static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_sweep),
+                    static_cast<void*>(d_as_ccol), static_cast<void*>(d_as_unit),
                     static_cast<void*>(d_as_unitw), static_cast<void*>(d_bt_rec), d_bt})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
