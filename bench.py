@@ -200,12 +200,121 @@ def measure_traffic(args, form, schedule, kclass, c_nelem):
     return res
 
 
+SERVE_EXPR = {2: "ik,kl->il", 3: "ijk,kl->ijl", 4: "ijmk,kl->ijml"}
+
+
+def serve_bench(args):
+    """Mixed request stream through the serving loop (continuous batching,
+    admission by footprint, merged launches): requests of order 2-4 (the last
+    mode of A contracted with B[k,l]), mode length L in [128, 1024], uniform
+    block size b in {8,16,24,32}, occupancy p ~ U(0.05, 0.5), FP64 or FP32,
+    arriving as a Poisson process of rate lambda.  A request whose expected
+    footprint (A + B + C bytes) exceeds the budget is rejected by the same rule
+    the server applies at submit, before its operands would be generated.
+    Admitted requests are generated once (seeded, on the GPU) and replayed for
+    every rate; latency = submit to completion."""
+    import shtensor_amd as sht
+    rng = np.random.default_rng(1004)
+    budget = int(float(os.environ.get("SHT_SERVE_BUDGET_GB", "200")) * (1 << 30))
+    gen_cap = budget // 2  # operands generated for the stream stay within half the budget
+    ctx = sht.Context(budget + (16 << 30))
+    reqs, rejected, gen_bytes = [], 0, 0
+    for r in range(args.serve_requests):
+        d = int(rng.choice([2, 3, 4]))
+        b = int(rng.choice([8, 16, 24, 32]))
+        L = int(rng.integers(128, 1025)) // b * b
+        p = float(rng.uniform(0.05, 0.5))
+        prec = sht.PREC_FP64 if rng.random() < 0.5 else sht.PREC_FP32
+        nb = L // b
+        a_el = p * nb ** d * b ** d
+        b_el = p * nb ** 2 * b ** 2
+        c_el = min(1.0, p * nb * p) * nb ** d * b ** d  # C block present if any k matches
+        foot = 8.0 * (a_el + b_el + c_el)
+        if foot > budget:
+            rejected += 1
+            continue
+        if gen_bytes + foot > gen_cap:
+            continue
+        gen_bytes += foot
+        expr = SERVE_EXPR[d]
+        ea, rest = expr.split(",")
+        eb, ec = rest.split("->")
+        bs = {ch: [b] * nb for ch in set(expr) if ch.isalpha()}
+        A = sht.Tensor(ctx, [bs[x] for x in ea])
+        B = sht.Tensor(ctx, [bs[x] for x in eb])
+        A.reserve_random(2000 + r, 0, p)
+        B.reserve_random(2000 + r, 1, p)
+        A.fill_random(2000 + r, 0, 0.0)
+        B.fill_random(2000 + r, 1, 0.0)
+        reqs.append({"expr": expr, "A": A, "B": B, "cbs": [bs[x] for x in ec], "prec": prec, "d": d, "L": L,
+                     "b": b, "p": p})
+    sht.call("sht_device_synchronize")
+    log("serve: %d requests admitted and generated (%.1f GB), %d rejected by footprint" % (
+        len(reqs), gen_bytes / 1e9, rejected))
+    # useful flops of each request (one solo run, also a warm-up of every shape's path)
+    for q in reqs:
+        Ct = sht.Tensor(ctx, q["cbs"])
+        pl = sht.Plan(ctx, q["expr"], q["A"], q["B"], Ct, q["prec"])
+        q["flops"] = pl.stats()["flops"]
+        pl.execute(1.0, 0.0)
+        sht.call("sht_device_synchronize")
+        pl.close()
+        Ct.close()
+    total_flops = sum(q["flops"] for q in reqs)
+    rates = [float(x) for x in args.serve_rates.split(",")]
+    runs = []
+    for lam in rates:
+        srv = sht.Server(ctx, budget, max_batch=64)
+        arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / lam, len(reqs)))
+        outs, ids = [], []
+        t0 = time.perf_counter()
+        for q, ta in zip(reqs, arr):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            Ct = sht.Tensor(ctx, q["cbs"])
+            outs.append(Ct)
+            ids.append(srv.submit(q["expr"], 1.0, q["A"], q["B"], 0.0, Ct, q["prec"]))
+        lat = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        for Ct in outs:
+            Ct.close()
+        lat = np.array(lat)
+        runs.append({"lambda_per_s": lam, "requests": len(reqs), "wall_s": round(wall, 3),
+                     "gflops": round(total_flops / wall / 1e9, 3),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "batches": st["batches"], "max_merged": st["max_merged"], "launches": st["launches"]})
+        log("serve lambda=%g/s: %s" % (lam, runs[-1]))
+    best = max(runs, key=lambda x: x["gflops"])
+    out = {
+        "metric": "contraction GFLOP/s FP64 vs batched/FP32 at 1e-6 rel err; p50/p99 latency",
+        "value": best["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": len(reqs), "warmup": 1,
+        "ms_per_step": round(1e3 * best["wall_s"] / max(1, len(reqs)), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic (seeded SplitMix64)",
+        "config": {"workload": "c4 mixed request stream (orders 2-4, L 128-1024, b 8-32, p 0.05-0.5, FP64/FP32)",
+                   "admitted": len(reqs), "rejected_by_footprint": rejected, "budget_bytes": budget,
+                   "useful_flops": total_flops},
+        "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"]},
+        "serve": runs, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for q in reqs:
+        q["A"].close()
+        q["B"].close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--serve-requests", type=int, default=200, help="c4: requests drawn for the stream")
+    ap.add_argument("--serve-rates", default="50,200,1000", help="c4: Poisson arrival rates (requests/s)")
     ap.add_argument("--length", type=int, default=None, help="override every mode length (smaller runs)")
     ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3",
                     help="fp64 (autotuned schedule), fp64_cs, fp64_as, fp64_as1, fp64_exact, fp32, bf16x3; _cs = C-stationary, _as = A-stationary 4-wave, _as1 = A-stationary one-wave units")
@@ -227,6 +336,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "c4":
+        serve_bench(args)
+        return
     import shtensor_amd as sht
 
     t0 = time.time()
