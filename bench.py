@@ -457,10 +457,22 @@ def main():
     # dominant kernel roofline (HIP events around each grouped launch)
     dom = max(r["launches"], key=lambda x: x[1])
     tot_ms = sum(ms for _, ms in r["launches"])
-    roof = {"bound": "mfma", "kernel": dom[0],
-            "achieved": round(r["stats"]["flops"] * (dom[1] / tot_ms) / (dom[1] * 1e-3) / 1e12, 3),
-            "peak": round(peak_of(main_f) / 1e12, 1), "unit": "TFLOP/s", "traffic": None}
+    # The bound follows the workload's algorithmic intensity: flops per
+    # algorithmic byte below the ridge (peak / 8 TB/s) is HBM-bound.  The
+    # dominant launch is credited with its share (by time) of the step's work.
+    share = dom[1] / tot_ms
+    alg_bytes = r["stats"].get("bytes") or 0.0
+    ai = r["stats"]["flops"] / alg_bytes if alg_bytes else float("inf")
+    if ai < peak_of(main_f) / HBM_PEAK:
+        roof = {"bound": "hbm", "kernel": dom[0],
+                "achieved": round(alg_bytes * share / (dom[1] * 1e-3) / 1e9, 1),
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "traffic": None}
+    else:
+        roof = {"bound": "mfma", "kernel": dom[0],
+                "achieved": round(r["stats"]["flops"] * share / (dom[1] * 1e-3) / 1e12, 3),
+                "peak": round(peak_of(main_f) / 1e12, 1), "unit": "TFLOP/s", "traffic": None}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    roof["intensity_flop_per_byte"] = round(ai, 2)
     out = {
         "metric": "contraction GFLOP/s FP64 vs batched/FP32 at 1e-6 rel err; p50/p99 latency",
         "value": round(value, 3),
