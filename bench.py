@@ -37,8 +37,10 @@ HBM_PEAK = 8.0e12
 # kernel class reported by sht_plan_execute_timed -> substring of the kernel's
 # symbol in rocprofv3's output
 KERNEL_SYMBOL = {
-    "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asp<2, 8, 0, 4, true>",
-    "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asp<2, 8, 1, 4, false>", "as_bf16x3": "k_as<2, 4, 2, 1>", "as_bf16x3_g": "k_as<2, 4, 2, 4>",
+    "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asl<0, 4, true, 8>",
+    "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asl<1, 4, false, 8>", "as_bf16x3": "k_asp<2, 8, 2, 1, false>",
+    "as_bf16x3_g": "k_asp<2, 8, 2, 4, false>", "as_mfma64_n4": "k_asp<2, 4, 0, 1, true>",
+    "as_mfma32_n4": "k_asp<2, 4, 1, 1, false>",
     "as_exact": "k_as_exact", "mfma64_n16": "k_mfma64<1>", "mfma64_n32": "k_mfma64<2>",
     "mfma32_n16": "k_mfma32<1>", "mfma32_n32": "k_mfma32<2>", "bf16x3_n16": "k_bf16x3<1>",
     "bf16x3_n32": "k_bf16x3<2>", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
@@ -317,7 +319,9 @@ def main():
     ap.add_argument("--serve-rates", default="50,200,1000", help="c4: Poisson arrival rates (requests/s)")
     ap.add_argument("--length", type=int, default=None, help="override every mode length (smaller runs)")
     ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3",
-                    help="fp64 (autotuned schedule), fp64_cs, fp64_as, fp64_as1, fp64_exact, fp32, bf16x3; _cs = C-stationary, _as = A-stationary 4-wave, _as1 = A-stationary one-wave units")
+                    help="fp64 (autotuned schedule), fp64_cs, fp64_as1, fp64_as2, fp64_as3, fp64_exact, fp32, "
+                         "bf16x3; _cs = C-stationary (schedule 0), _asN = A-stationary schedule N (1: one-wave "
+                         "units, 2: four units of a block row per workgroup, 3: narrow sweeps)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
@@ -342,11 +346,11 @@ def main():
     import shtensor_amd as sht
 
     t0 = time.time()
-    precs = {"fp64": sht.PREC_FP64, "fp64_cs": sht.PREC_FP64, "fp64_as": sht.PREC_FP64, "fp64_as1": sht.PREC_FP64,
-             "fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_cs": sht.PREC_FP64_EXACT,
-             "fp64_exact_as": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "fp32_cs": sht.PREC_FP32,
-             "fp32_as": sht.PREC_FP32, "fp32_as1": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3, "bf16x3_cs": sht.PREC_BF16X3,
-             "bf16x3_as": sht.PREC_BF16X3, "bf16x3_as1": sht.PREC_BF16X3}
+    base = {"fp64": sht.PREC_FP64, "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
+    precs = dict(base)
+    for b, pr in base.items():
+        for suf in ("_cs", "_as1", "_as2", "_as3"):
+            precs[b + suf] = pr
     forms = [f for f in args.forms.split(",") if f]
     arena = int(float(os.environ.get("SHT_BENCH_ARENA_GB", "200")) * (1 << 30))
     ctx = sht.Context(arena)
@@ -366,10 +370,8 @@ def main():
             plan.set_schedule(args.schedule)
         elif f in ("fp64", "fp64_exact", "fp32", "bf16x3"):
             tuned = plan.autotune(1.0, 0.0, reps=2, stream=stream)  # picks the schedule
-        elif f.endswith("_as"):
-            plan.set_schedule(1)
-        elif f.endswith("_as1"):
-            plan.set_schedule(2)
+        elif "_as" in f:
+            plan.set_schedule(int(f[-1]))
         st = plan.stats()
         st["schedule"] = plan.get_schedule()
         st["autotune_ms"] = tuned

@@ -163,15 +163,19 @@ sht_status sht_plan_execute_timed(sht_plan* p, double alpha, double beta, void* 
 /* Name of a kernel class reported by sht_plan_execute_timed.               */
 const char* sht_kclass_name(int kclass);
 /* Execution schedule: 0 = C-stationary tiles (one tile per workgroup, C
- * block accumulated in registers over its tasks), 1 = A-stationary row
- * panels in 4-wave workgroups (each A element read once per 16-row chunk),
- * 2 = A-stationary row panels in one-wave workgroups (for SHT_PREC_FP64_EXACT
- * the same kernel as 1).                                                   */
+ * block accumulated in registers over its tasks); 1 = A-stationary row
+ * panels, one wave (one 32-row chunk of a block row) per workgroup, each A
+ * element read once per sweep of at most 8 16-column slots; 2 = the same with
+ * the chunks of one block row grouped four to a workgroup, sharing the panel
+ * (not for SHT_PREC_FP64_EXACT); 3 = schedule 1 over sweeps of at most 4
+ * slots at three waves per SIMD (SHT_PREC_FP64 and SHT_PREC_FP32 only).
+ * Schedules 1-3 need every A block below 2^28 elements.  Results do not
+ * depend on the schedule (FP64 forms: bit for bit).                         */
 sht_status sht_plan_set_schedule(sht_plan* p, int schedule);
 sht_status sht_plan_get_schedule(const sht_plan* p, int* schedule);
 /* Autotuner: time each applicable schedule (`reps` executions on `stream`)
- * and keep the fastest.  ms_out (3 entries, may be NULL) receives the time
- * of schedules 0, 1 and 2 (-1 where not tried).
+ * and keep the fastest.  ms_out (4 entries, may be NULL) receives the time
+ * of schedules 0-3 (-1 where not tried).
  * C is overwritten by the trial executions.                                 */
 sht_status sht_plan_autotune(sht_plan* p, double alpha, double beta, int reps, void* stream, float* ms_out);
 /* capture one execution into a hipGraph and replay it `replays` times       */

@@ -352,7 +352,9 @@ class Plan:
         return [(lib().sht_kclass_name(kc[i]).decode(), float(ms[i])) for i in range(min(n.value, 64))]
 
     def set_schedule(self, schedule):
-        """0: C-stationary tiles, 1: A-stationary row panels (FP64 only)."""
+        """0: C-stationary tiles; 1: A-stationary, one-wave units; 2: A-stationary,
+        four units of a block row per workgroup; 3: A-stationary, narrow sweeps
+        (FP64 / FP32)."""
         call("sht_plan_set_schedule", C.c_void_p(self.ptr), int(schedule))
 
     def get_schedule(self):
@@ -362,9 +364,9 @@ class Plan:
 
     def autotune(self, alpha=1.0, beta=0.0, reps=2, stream=None):
         """Time every applicable schedule, keep the fastest; {schedule: ms}."""
-        ms = (C.c_float * 3)(-1.0, -1.0, -1.0)
+        ms = (C.c_float * 4)(-1.0, -1.0, -1.0, -1.0)
         call("sht_plan_autotune", C.c_void_p(self.ptr), float(alpha), float(beta), int(reps), _sptr(stream), ms)
-        return {i: float(ms[i]) for i in range(3) if ms[i] >= 0}
+        return {i: float(ms[i]) for i in range(4) if ms[i] >= 0}
 
     def tasks(self):
         ptrs = [C.POINTER(C.c_int64)() for _ in range(4)]

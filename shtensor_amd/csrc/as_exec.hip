@@ -281,6 +281,233 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
 }
 
 // ---------------------------------------------------------------------------
+// The same sweep for a workgroup of W waves that run W row chunks of one
+// block row through the same sweep list: each k-step's panel fragments are
+// fetched once per workgroup (wave w loads the slots q with q % W == w) and
+// shared through LDS (two stages, one workgroup barrier per step), so every
+// wave reads all NS fragments from LDS instead of from L1/L2.  Per step s:
+// issue this wave's panel loads for step s+2, A loads for step s+3, read
+// step s's fragments from LDS, run the MFMAs, write this wave's share of
+// step s+1 (loaded one step earlier) into the other stage, barrier.  A ghost
+// wave (a padding unit of the workgroup) only takes part in the panel loads
+// and barriers.
+// ---------------------------------------------------------------------------
+template <int NS, int F, bool A1, int W, int AD>
+__device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU& nxs, int r0, int li, int lk, int wave,
+                                 bool ghost, double (&ar)[AD][2], const int64_t* cbs,
+                                 typename BElem<F>::type (*stage)[kAsSweepSlots][64])
+{
+  constexpr int R = 2;
+  constexpr int NSW = (NS + W - 1) / W;  // slots this wave loads (at most)
+  typedef typename BElem<F>::type TB;
+  typedef typename std::conditional<F == 0, v4d, v4f>::type Acc;
+  const int lane = li + 16 * lk;
+  const TB* Bl = static_cast<const TB*>(p.B) + sw.b_off + li + static_cast<int64_t>(lk) * sw.pitch;
+  const int64_t bstep = 4 * static_cast<int64_t>(sw.pitch);
+  const int K = sw.k, M = sw.m;
+  double* C = static_cast<double*>(p.C);
+  auto colof = [&](int q, int v) { return F == 0 ? 16 * q + 4 * v + lk : 16 * q + 4 * lk + v; };
+  const int rl = r0 + li;
+  // AD steps per loop iteration (the A ring's size); steps past the panel's
+  // rows are masked (k >= K) and re-read its last row
+  const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
+  const int nstepsU = (nsteps + AD - 1) / AD * AD;
+  // Every global load below is unconditional (ghost waves included, whose
+  // row chunk is chunk 0 of the block row): a load under a branch makes the
+  // compiler drain all outstanding loads where the paths merge, which would
+  // discard the A and panel prefetch.  Ghosts skip only MFMAs and stores.
+  Acc acc[R][NS];
+  if constexpr (F == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t cb = cbs[colof(q, v)];
+        const double* cq = C + (cb >= 0 ? cb : 0) + rl;
+#pragma unroll
+        for (int t = 0; t < R; ++t) acc[t][q][v] = cq[16 * t];
+      }
+  } else {
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  // this wave's share of the panel: slots wave, wave + W, ...; a share past
+  // the sweep's slots re-reads slot NS-1 into an unused stage slot
+  static_assert(W * ((kAsSweepSlots + W - 1) / W) <= kAsSweepSlots, "stage slots");
+  auto loadS = [&](int st, TB(&g)[NSW]) {
+    const TB* bs = Bl + min(st, last) * bstep;
+#pragma unroll
+    for (int x = 0; x < NSW; ++x) g[x] = bs[16 * min(wave + W * x, NS - 1)];
+  };
+  auto writeS = [&](int stg, const TB(&g)[NSW]) {
+#pragma unroll
+    for (int x = 0; x < NSW; ++x) stage[stg][wave + W * x][lane] = g[x];
+  };
+  // this wave's share of steps s+1 .. s+BD of the panel, in flight in a
+  // ring of BD register sets (set s % BD)
+  constexpr int BD = 4;
+  TB g[BD][NSW];
+#pragma unroll
+  for (int i = 0; i < BD; ++i) loadS(i, g[i]);
+  writeS(0, g[0]);
+  __syncthreads();
+  AStream cu, nx;
+  cu.a = static_cast<const double*>(p.A) + sw.a_off;
+  cu.K = K;
+  cu.M = M;
+  nx.a = static_cast<const double*>(p.A) + nxs.a_off;
+  nx.K = nxs.k;
+  nx.M = nxs.m;
+  TB b[NS];
+  auto mma = [&](int st, const double(&araw)[R]) {
+    const bool kin = 4 * st + lk < K;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if constexpr (F == 0) {
+        const double y = kin ? araw[t] : 0.0;
+        const double a = A1 ? y : y * p.alpha;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a, acc[t][q], 0, 0, 0);
+      } else {
+        const float a = kin ? static_cast<float>(araw[t]) : 0.f;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[q], a, acc[t][q], 0, 0, 0);
+      }
+    }
+  };
+  // one step: j = position in the 4-step iteration (ring indices are
+  // compile-time), gs = register set of this step's share (step st+j+1 is in
+  // the other set)
+  auto phase = [&](int st, auto jc) {
+    constexpr int j = decltype(jc)::value;
+    loadS(st + j + BD, g[j % BD]);  // the set held step st+j, already in LDS
+    load_a2(cu, nx, nstepsU, st + j + AD - 1, rl, lk, ar[(j + AD - 1) % AD]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (!ghost) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) b[q] = stage[j % 2][q][lane];
+      mma(st + j, ar[j]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    writeS((j + 1) % 2, g[(j + 1) % BD]);
+    __syncthreads();
+  };
+  static_assert(AD % BD == 0, "ring sizes");
+  for (int st = 0; st < nstepsU; st += AD) {
+    phase(st, std::integral_constant<int, 0>());
+    phase(st, std::integral_constant<int, 1>());
+    phase(st, std::integral_constant<int, 2>());
+    phase(st, std::integral_constant<int, 3>());
+    if constexpr (AD == 8) {
+      phase(st, std::integral_constant<int, 4>());
+      phase(st, std::integral_constant<int, 5>());
+      phase(st, std::integral_constant<int, 6>());
+      phase(st, std::integral_constant<int, 7>());
+    }
+  }
+  // write back (as in sweep_mfma; ghosts store nothing)
+  if constexpr (F == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t cb = cbs[colof(q, v)];
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+          if (!ghost && cb >= 0 && rl + 16 * t < M) C[cb + rl + 16 * t] = acc[t][q][v];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    constexpr int G = NS < 2 ? NS : 2;
+#pragma unroll
+    for (int q0 = 0; q0 < NS; q0 += G) {
+      int64_t cb[G][4];
+      double cv[G][4][R];
+#pragma unroll
+      for (int x = 0; x < G; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          cb[x][v] = q0 + x < NS ? cbs[colof(q0 + x, v)] : -1;
+          const double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
+#pragma unroll
+          for (int t = 0; t < R; ++t) cv[x][v][t] = cq[16 * t];
+        }
+#pragma unroll
+      for (int x = 0; x < G; ++x) {
+        if (q0 + x >= NS) break;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
+#pragma unroll
+          for (int t = 0; t < R; ++t)
+            if (!ghost && cb[x][v] >= 0 && rl + 16 * t < M)
+              cq[16 * t] = cv[x][v][t] + p.alpha * static_cast<double>(acc[t][q0 + x][v]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Workgroups of W waves over W row chunks of one block row (the LDS-shared
+// panel path; F: 0 = FP64 MFMA, 1 = FP32 MFMA).  Every wave of a workgroup
+// runs the same sweep list, ghosts included, so the workgroup barriers match.
+template <int F, int W, bool A1, int AD>
+__global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
+{
+  static_assert(AD == 4 || AD == 8, "A ring of 4 or 8 steps");
+  typedef typename BElem<F>::type TB;
+  const int wave = uni(static_cast<int>(threadIdx.x >> 6));
+  const int64_t ui = static_cast<int64_t>(blockIdx.x) * W;
+  if (ui >= p.nunits) return;  // (the unit list is padded to whole workgroups)
+  const AsUnit u = p.units[ui + wave];
+  const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
+  const bool ghost = uni(u.pad) != 0;
+  if (nsw == 0) return;  // a whole-workgroup filler (every wave of it)
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  __shared__ int64_t cbs_all[W][128];
+  __shared__ TB stage[2][kAsSweepSlots][64];
+  int64_t* cbs = cbs_all[wave];
+  SweepU cur = to_uni(p.sweeps[s0]);
+  SweepU nxt = to_uni(p.sweeps[s0 + min(1, nsw - 1)]);
+  AsSweep raw2 = p.sweeps[s0 + min(2, nsw - 1)];
+  int64_t cc0, cc1;
+  load_cols(p, cur, lane, cc0, cc1);
+  double ar[AD][2];
+  {
+    AStream s;
+    s.a = static_cast<const double*>(p.A) + cur.a_off;
+    s.K = cur.k;
+    s.M = cur.m;
+#pragma unroll
+    for (int i = 0; i < AD - 1; ++i) load_a(s, i, r0 + li, lk, ar[i]);
+  }
+  for (int j = 0; j < nsw; ++j) {
+    cbs[lane] = cc0;
+    cbs[lane + 64] = cc1;
+    wave_sync();
+    load_cols(p, nxt, lane, cc0, cc1);
+    switch (cur.ns) {
+      case 1: sweep_lds<1, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 2: sweep_lds<2, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 3: sweep_lds<3, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 4: sweep_lds<4, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 5: sweep_lds<5, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 6: sweep_lds<6, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 7: sweep_lds<7, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      default: sweep_lds<8, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+    }
+    wave_sync();
+    cur = nxt;
+    nxt = to_uni(raw2);
+    raw2 = p.sweeps[s0 + min(j + 3, nsw - 1)];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Three-term split-BF16 sweep.  x (rounded to FP32) = hi + mid + lo, each a
 // bf16 (round to nearest even); per 32-k step six products of order <= 2,
 // smallest first: lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi.  The raw A
@@ -403,7 +630,8 @@ __device__ inline void run_sweep(const AsArgs& p, const SweepU& sw, const SweepU
 // The records of the next two sweeps and the next sweep's column offsets are
 // loaded one sweep ahead, so a sweep starts without a dependent load chain.
 template <int R, int NSMAX, int F, int WPG, bool A1>
-__global__ __launch_bounds__(64 * WPG) void k_asp(AsArgs p)
+__global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(NSMAX <= kAsNarrowSlots ? 3 : 1)))
+void k_asp(AsArgs p)
 {
   static_assert(16 * NSMAX <= 128, "two column offsets per lane");
   const int wave = WPG == 1 ? 0 : uni(static_cast<int>(threadIdx.x >> 6));
@@ -411,7 +639,7 @@ __global__ __launch_bounds__(64 * WPG) void k_asp(AsArgs p)
   if (ui >= p.nunits) return;
   const AsUnit u = p.units[ui];
   const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
-  if (nsw == 0) return;
+  if (nsw == 0 || uni(u.pad) != 0) return;  // empty or ghost unit
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   __shared__ int64_t cbs_all[WPG][128];
   int64_t* cbs = cbs_all[wave];
@@ -452,6 +680,7 @@ __global__ __launch_bounds__(64 * WPG) void k_asp(AsArgs p)
   }
 }
 
+#ifndef SHT_AS_BF3_TU
 // ---------------------------------------------------------------------------
 // A-stationary fma chain (bitwise equal to the oracle).  One wave per 64-row
 // unit, lane = one C row; per chunk of 64 sweep columns (kAsxSlots slots of
@@ -555,8 +784,30 @@ __global__ __launch_bounds__(256) void k_scale(double* c, int64_t n, double beta
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
     c[i] = bz ? 0.0 : c[i] * beta;
 }
+#endif  // !SHT_AS_BF3_TU
 
 }  // namespace
+
+// This file is compiled twice: the FP64 / FP32 / fma-chain kernels with
+// VGPR-form MFMAs (accumulators in ordinary VGPRs: with AGPR accumulators
+// the compiler copies them between register files inside the k loops), and,
+// with SHT_AS_BF3_TU defined, the split-BF16 kernels with AGPR accumulators
+// (their VGPR form needs more than 256 VGPRs: one wave per SIMD).
+#ifdef SHT_AS_BF3_TU
+
+void launch_as_bf3(AsClass k, const AsArgs& a, hipStream_t s)
+{
+  constexpr int G = kAsGroupWaves;
+  const dim3 g1(static_cast<unsigned>(a.nunits)), b1(64);
+  const dim3 gg(static_cast<unsigned>((a.nunits + G - 1) / G)), bg(64 * G);
+  switch (k) {
+    case AsClass::Bf16x3: hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false>), g1, b1, 0, s, a); break;
+    case AsClass::Bf16x3G: hipLaunchKernelGGL((k_asp<2, 8, 2, G, false>), gg, bg, 0, s, a); break;
+    default: throw Error(SHT_ERR_INVALID, "not a split-BF16 class");
+  }
+}
+
+#else
 
 int as_group_waves(AsClass k)
 {
@@ -576,19 +827,26 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
   const dim3 gg(static_cast<unsigned>((a.nunits + G - 1) / G)), bg(64 * G);
   static_assert(kAsUnitRows == 32 && kAsUnitRowsW == 64, "unit rows: two MFMA tiles; 64 lanes");
   static_assert(kAsSweepSlots == 8, "sweep slots");
+  // the grouped classes read W unit records per workgroup
+  SHT_REQUIRE(as_group_waves(k) == 1 || a.nunits % G == 0, SHT_ERR_STATE, "grouped unit list not padded");
   switch (k) {
     case AsClass::Mfma64:
       if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, 8, 0, 1, true>), g1, b1, 0, s, a);
       else hipLaunchKernelGGL((k_asp<2, 8, 0, 1, false>), g1, b1, 0, s, a);
       break;
     case AsClass::Mfma64G:
-      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, 8, 0, G, true>), gg, bg, 0, s, a);
-      else hipLaunchKernelGGL((k_asp<2, 8, 0, G, false>), gg, bg, 0, s, a);
+      if (a.alpha_one) hipLaunchKernelGGL((k_asl<0, G, true, 8>), gg, bg, 0, s, a);
+      else hipLaunchKernelGGL((k_asl<0, G, false, 8>), gg, bg, 0, s, a);
       break;
     case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Mfma32G: hipLaunchKernelGGL((k_asp<2, 8, 1, G, false>), gg, bg, 0, s, a); break;
-    case AsClass::Bf16x3: hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Bf16x3G: hipLaunchKernelGGL((k_asp<2, 8, 2, G, false>), gg, bg, 0, s, a); break;
+    case AsClass::Mfma32G: hipLaunchKernelGGL((k_asl<1, G, false, 8>), gg, bg, 0, s, a); break;
+    case AsClass::Bf16x3:
+    case AsClass::Bf16x3G: launch_as_bf3(k, a, s); break;
+    case AsClass::Mfma64N:
+      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, kAsNarrowSlots, 0, 1, true>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<2, kAsNarrowSlots, 0, 1, false>), g1, b1, 0, s, a);
+      break;
+    case AsClass::Mfma32N: hipLaunchKernelGGL((k_asp<2, kAsNarrowSlots, 1, 1, false>), g1, b1, 0, s, a); break;
     case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g1, b1, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }
@@ -623,5 +881,7 @@ void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStrea
   hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, s, c, nelem, beta, beta_zero);
   SHT_HIP(hipGetLastError());
 }
+
+#endif  // SHT_AS_BF3_TU
 
 }  // namespace sht

@@ -619,16 +619,15 @@ void build_as_schedule(sht_plan* P)
     panels.push_back(pn);
     u = e;
   }
-  // sweep list per block row, with the C offset of every sweep column
-  std::vector<AsSweep> sweeps;
-  std::vector<int64_t> ccol;
-  std::vector<int64_t> pcof;  // partner C offsets of the current A block
+  // sweep list per block row (sweeps of at most `cap` slots), with the C
+  // offset of every sweep column
   struct Row {
     int32_t sweep0, nsweep;
     int64_t rows;
     double work;  // sum over sweeps of k * columns
   };
-  std::vector<Row> brows;
+  auto gen_sweeps = [&](int cap, std::vector<AsSweep>& sweeps, std::vector<int64_t>& ccol, std::vector<Row>& brows) {
+  std::vector<int64_t> pcof;  // partner C offsets of the current A block
   int64_t t = 0;
   while (t < nt) {
     const int64_t I = aI[ta[order[t]]];
@@ -643,7 +642,7 @@ void build_as_schedule(sht_plan* P)
       }
       const int32_t k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
       const int nslots = (pn.ncol + kBtPad - 1) / kBtPad;
-      for (int g = 0; g < nslots; g += kAsSweepSlots) {
+      for (int g = 0; g < nslots; g += cap) {
         AsSweep sw{};
         sw.a_off = A->offsets[a];
         sw.b_off = bf3 ? pn.bt_off + static_cast<int64_t>(16 * g) * pn.pitch : pn.bt_off + 16 * g;
@@ -651,7 +650,7 @@ void build_as_schedule(sht_plan* P)
         sw.k = k;
         sw.m = static_cast<int32_t>(row.rows);
         sw.pitch = pn.pitch;
-        sw.ns = std::min(kAsSweepSlots, nslots - g);
+        sw.ns = std::min(cap, nslots - g);
         for (int c = 16 * g; c < 16 * (g + sw.ns); ++c) {
           int64_t v = -1;
           if (c < pn.ncol) {
@@ -668,6 +667,14 @@ void build_as_schedule(sht_plan* P)
     brows.push_back(row);
   }
   SHT_REQUIRE(sweeps.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
+  };
+  std::vector<AsSweep> sweeps, sweeps4;
+  std::vector<int64_t> ccol, ccol4;
+  std::vector<Row> brows, brows4;
+  gen_sweeps(kAsSweepSlots, sweeps, ccol, brows);
+  // narrow sweeps (schedule 3): FP64 / FP32 MFMA only
+  const bool narrow = !exact && !bf3;
+  if (narrow) gen_sweeps(kAsNarrowSlots, sweeps4, ccol4, brows4);
   // XCD-aware order.  Workgroup g runs on XCD g % 8 (round-robin dispatch),
   // and every XCD has its own L2: all units of a block row go to one XCD, so
   // the row's panels are shared in that L2.  Rows are dealt to the XCD with
@@ -675,10 +682,10 @@ void build_as_schedule(sht_plan* P)
   // interleaved; short queues are padded with empty units.
   constexpr int kXcd = 8;
   // Units of one block row are grouped per_wg to a workgroup (padded with
-  // empty units, so a workgroup never mixes rows): the waves of a workgroup
-  // run the same pass sequence on neighbouring row chunks and share the
-  // panel fragments through the CU's L1.
-  auto build = [&](int step, int per_wg) {
+  // ghost units, so a workgroup never mixes rows): the waves of a workgroup
+  // run the same sweep list on neighbouring row chunks and share the panel
+  // fragments (through LDS for the FP64/FP32 classes, L1 for split-BF16).
+  auto build = [&](const std::vector<Row>& brows, int step, int per_wg) {
     std::vector<int64_t> ro(brows.size());
     std::iota(ro.begin(), ro.end(), 0);
     std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) {
@@ -691,7 +698,9 @@ void build_as_schedule(sht_plan* P)
       const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
       for (int64_t r0 = 0; r0 < br.rows; r0 += step)
         q[x].push_back(AsUnit{br.sweep0, br.nsweep, static_cast<int32_t>(r0), 0});
-      while (q[x].size() % per_wg) q[x].push_back(AsUnit{0, 0, 0, 0});
+      // ghost units: pad the row's workgroup; they walk the same sweep list
+      // (shared panel loads and barriers) without rows of their own
+      while (q[x].size() % per_wg) q[x].push_back(AsUnit{br.sweep0, br.nsweep, 0, 1});
       load[x] += br.work * static_cast<double>(br.rows);
     }
     size_t longest = 0;
@@ -706,8 +715,9 @@ void build_as_schedule(sht_plan* P)
     while (out.size() % per_wg) out.push_back(AsUnit{0, 0, 0, 0});
     return out;
   };
-  std::vector<AsUnit> us = build(exact ? kAsUnitRowsW : kAsUnitRows, 1);
-  std::vector<AsUnit> usw = exact ? std::vector<AsUnit>() : build(kAsUnitRows, kAsGroupWaves);
+  std::vector<AsUnit> us = build(brows, exact ? kAsUnitRowsW : kAsUnitRows, 1);
+  std::vector<AsUnit> usw = exact ? std::vector<AsUnit>() : build(brows, kAsUnitRows, kAsGroupWaves);
+  std::vector<AsUnit> us4 = narrow ? build(brows4, kAsUnitRows, 1) : std::vector<AsUnit>();
   Arena& arena = P->ctx->arena;
   auto upload = [&](const void* src, size_t bytes) {
     void* d = arena.allocate(std::max<size_t>(bytes, 1));
@@ -718,6 +728,11 @@ void build_as_schedule(sht_plan* P)
   P->d_as_ccol = static_cast<int64_t*>(upload(ccol.data(), ccol.size() * sizeof(int64_t)));
   P->d_as_unit = static_cast<AsUnit*>(upload(us.data(), us.size() * sizeof(AsUnit)));
   P->d_as_unitw = static_cast<AsUnit*>(upload(usw.data(), usw.size() * sizeof(AsUnit)));
+  if (narrow) {
+    P->d_as_sweep4 = static_cast<AsSweep*>(upload(sweeps4.data(), sweeps4.size() * sizeof(AsSweep)));
+    P->d_as_ccol4 = static_cast<int64_t*>(upload(ccol4.data(), ccol4.size() * sizeof(int64_t)));
+    P->d_as_unit4 = static_cast<AsUnit*>(upload(us4.data(), us4.size() * sizeof(AsUnit)));
+  }
   P->d_bt_rec = static_cast<BtRec*>(upload(bt.data(), bt.size() * sizeof(BtRec)));
   const size_t bt_bytes = bf3 ? 3 * sizeof(uint16_t) : (P->prec == SHT_PREC_FP32 ? sizeof(float) : sizeof(double));
   P->d_bt = arena.allocate(std::max<int64_t>(bt_total, 1) * bt_bytes);
@@ -729,6 +744,7 @@ void build_as_schedule(sht_plan* P)
   P->bt_nrec = static_cast<int64_t>(bt.size());
   P->as_nunits = static_cast<int64_t>(us.size());
   P->as_nunitsw = static_cast<int64_t>(usw.size());
+  P->as_nunits4 = static_cast<int64_t>(us4.size());
   P->as_nsweep = static_cast<int64_t>(sweeps.size());
   P->as_npanel = static_cast<int64_t>(panels.size());
   P->as_built = true;
@@ -755,14 +771,17 @@ BtFormat bt_format(const sht_plan* P)
 
 AsClass as_class(const sht_plan* P)
 {
-  const bool wide = P->schedule == 2;
+  const bool wide = P->schedule == 2, narrow = P->schedule == 3;
   switch (P->prec) {
     case SHT_PREC_FP64_EXACT: return AsClass::Exact;
-    case SHT_PREC_FP32: return wide ? AsClass::Mfma32G : AsClass::Mfma32;
+    case SHT_PREC_FP32: return wide ? AsClass::Mfma32G : (narrow ? AsClass::Mfma32N : AsClass::Mfma32);
     case SHT_PREC_BF16X3: return wide ? AsClass::Bf16x3G : AsClass::Bf16x3;
-    default: return wide ? AsClass::Mfma64G : AsClass::Mfma64;
+    default: return wide ? AsClass::Mfma64G : (narrow ? AsClass::Mfma64N : AsClass::Mfma64);
   }
 }
+
+// schedule 3 (narrow sweeps) exists for the FP64 and FP32 MFMA forms
+bool narrow_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32; }
 
 void setup_staging(sht_plan* P)
 {
@@ -836,12 +855,12 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.B = P->d_bt;
       g.bt_total = P->bt_total;
       g.C = Cd;
-      g.sweeps = P->d_as_sweep;
-      g.ccol = P->d_as_ccol;
       const AsClass cls = as_class(P);
-      const bool wide = as_group_waves(cls) > 1;
-      g.units = wide ? P->d_as_unitw : P->d_as_unit;
-      g.nunits = wide ? P->as_nunitsw : P->as_nunits;
+      const bool wide = as_group_waves(cls) > 1, narrow = P->schedule == 3;
+      g.sweeps = narrow ? P->d_as_sweep4 : P->d_as_sweep;
+      g.ccol = narrow ? P->d_as_ccol4 : P->d_as_ccol;
+      g.units = narrow ? P->d_as_unit4 : (wide ? P->d_as_unitw : P->d_as_unit);
+      g.nunits = narrow ? P->as_nunits4 : (wide ? P->as_nunitsw : P->as_nunits);
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
@@ -964,8 +983,8 @@ sht_status sht_plan_execute_timed(sht_plan* P, double alpha, double beta, void* 
 
 const char* sht_kclass_name(int k)
 {
-  static const char* as_names[] = {"as_mfma64", "as_exact", "as_mfma64_g", "as_mfma32",
-                                   "as_bf16x3", "as_mfma32_g", "as_bf16x3_g"};
+  static const char* as_names[] = {"as_mfma64",   "as_exact",    "as_mfma64_g",  "as_mfma32",   "as_bf16x3",
+                                   "as_mfma32_g", "as_bf16x3_g", "as_mfma64_n4", "as_mfma32_n4"};
   if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
   static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
                                 "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
@@ -1005,9 +1024,11 @@ sht_status sht_plan_graph_replay(sht_plan* P, double alpha, double beta, int rep
 sht_status sht_plan_set_schedule(sht_plan* P, int schedule)
 {
   return guard([&] {
-    SHT_REQUIRE(P && schedule >= 0 && schedule <= 2, SHT_ERR_INVALID, "schedule must be 0, 1 or 2");
+    SHT_REQUIRE(P && schedule >= 0 && schedule <= 3, SHT_ERR_INVALID, "schedule must be 0, 1, 2 or 3");
     if (schedule >= 1) {
       SHT_REQUIRE(sht::as_supported(P), SHT_ERR_UNSUPPORTED, "A-stationary schedule not available");
+      SHT_REQUIRE(schedule != 3 || sht::narrow_supported(P), SHT_ERR_UNSUPPORTED,
+                  "narrow A-stationary schedule not available for this precision");
       sht::build_as_schedule(P);
     }
     if (P->schedule != schedule && P->graph) {
@@ -1036,12 +1057,13 @@ sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, v
     SHT_REQUIRE(P && reps > 0, SHT_ERR_INVALID, "bad argument");
     hipStream_t s = sht::as_stream(stream);
     if (ms_out)
-      for (int c = 0; c < 3; ++c) ms_out[c] = -1.f;
+      for (int c = 0; c < 4; ++c) ms_out[c] = -1.f;
     std::vector<int> cand{0};
     if (sht::as_supported(P)) {
       cand.push_back(1);
       // the fma-chain form has a single A-stationary kernel
       if (P->prec != SHT_PREC_FP64_EXACT) cand.push_back(2);
+      if (sht::narrow_supported(P)) cand.push_back(3);
     }
     if (P->graph) {
       SHT_HIP(hipGraphExecDestroy(P->graph));

@@ -69,6 +69,7 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
 // its block and every C element is updated in place in the task order
 // (C id, K) of the task list.
 constexpr int kAsSweepSlots = 8;
+constexpr int kAsNarrowSlots = 4;  // slots per sweep of the narrow schedule (3)
 struct AsSweep {
   int64_t a_off;   // A block (m x k, column-major)
   int64_t b_off;   // first sweep column in the panel, in elements of the B format
@@ -102,7 +103,7 @@ struct AsUnit {
   int32_t sweep0;  // first sweep of the block row
   int32_t nsweep;  // sweeps of the block row (0: empty unit)
   int32_t r0;      // first row of the chunk
-  int32_t pad;
+  int32_t pad;     // 1: ghost unit (pads a grouped workgroup; no rows of its own)
 };
 struct AsArgs {
   const void* A;
@@ -119,15 +120,18 @@ struct AsArgs {
 // Kernel classes.  The MFMA classes run 32-row units (two 16-row MFMA tiles
 // per wave): Mfma64/Mfma32/Bf16x3 one wave per workgroup, the *G variants
 // kAsGroupWaves waves (units of one block row) per workgroup sharing panel
-// fragments through L1.  Exact runs 64-row units with one row per lane.
+// fragments (through LDS for FP64/FP32, through L1 for split-BF16).  Exact runs 64-row units with one row per lane.
+// The *N classes (schedule 3) run sweeps of at most kAsNarrowSlots slots at
+// three waves per SIMD.
 enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64G = 2, Mfma32 = 3, Bf16x3 = 4, Mfma32G = 5, Bf16x3G = 6,
-                           Count };
+                           Mfma64N = 7, Mfma32N = 8, Count };
 constexpr int kAsUnitRows = 32;    // rows of a unit of the MFMA kernels
 constexpr int kAsUnitRowsW = 64;   // rows of a unit of the fma-chain kernel
 constexpr int kAsGroupWaves = 4;   // waves per workgroup of the *G classes
 constexpr int kAsxCols = 8;        // columns per fma-chain slot
 constexpr int kAsxSlots = 8;       // slots per fma-chain sweep
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
+void launch_as_bf3(AsClass k, const AsArgs& a, hipStream_t s);  // the split-BF16 classes
 // waves per workgroup of a class (its units come grouped that way)
 int as_group_waves(AsClass k);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
@@ -181,16 +185,21 @@ struct sht_plan {
   double bytes_in = 0.0;  // A + B + C write
   double bytes_c = 0.0;   // C read (beta != 0)
   // A-stationary schedule (built on demand)
-  int schedule = 0;  // 0: C-stationary tiles, 1: A-stationary one-wave units, 2: A-stationary grouped units
+  // 0: C-stationary tiles, 1: A-stationary one-wave units, 2: A-stationary
+  // grouped units, 3: A-stationary one-wave units over narrow sweeps
+  int schedule = 0;
   bool as_built = false;
   sht::AsSweep* d_as_sweep = nullptr;
   int64_t* d_as_ccol = nullptr;
   sht::AsUnit* d_as_unit = nullptr;   // one unit per workgroup (32 rows; 64 for the fma chain)
   sht::AsUnit* d_as_unitw = nullptr;  // kAsGroupWaves units of one block row per workgroup
+  sht::AsSweep* d_as_sweep4 = nullptr;  // the narrow sweep list (kAsNarrowSlots), its columns and units
+  int64_t* d_as_ccol4 = nullptr;
+  sht::AsUnit* d_as_unit4 = nullptr;
   sht::BtRec* d_bt_rec = nullptr;
   void* d_bt = nullptr;
   int64_t bt_nrec = 0, bt_total = 0;
-  int64_t as_nunits = 0, as_nunitsw = 0, as_nsweep = 0, as_npanel = 0;
+  int64_t as_nunits = 0, as_nunitsw = 0, as_nunits4 = 0, as_nsweep = 0, as_npanel = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   double graph_alpha = 0.0, graph_beta = 0.0;
@@ -202,7 +211,8 @@ struct sht_plan {
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
                     static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_sweep),
                     static_cast<void*>(d_as_ccol), static_cast<void*>(d_as_unit),
-                    static_cast<void*>(d_as_unitw), static_cast<void*>(d_bt_rec), d_bt})
+                    static_cast<void*>(d_as_unitw), static_cast<void*>(d_as_sweep4), static_cast<void*>(d_as_ccol4),
+                    static_cast<void*>(d_as_unit4), static_cast<void*>(d_bt_rec), d_bt})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);

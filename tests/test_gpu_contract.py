@@ -22,23 +22,23 @@ CASES = [
     ("akbl,lcka->bc", "akbl", "lcka", "bc"),
     ("ijk,km->ijm", "ijk", "km", "ijm"),
 ]
-# suffix: none = C-stationary (schedule 0), _as = A-stationary 4-wave groups
-# (1), _as1 = A-stationary one-wave units (2)
-PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_as": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64,
-         "fp64_as": sht.PREC_FP64, "fp64_as1": sht.PREC_FP64, "fp32": sht.PREC_FP32, "fp32_as": sht.PREC_FP32,
-         "fp32_as1": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3, "bf16x3_as": sht.PREC_BF16X3,
-         "bf16x3_as1": sht.PREC_BF16X3}
-TOL = {"fp64_exact": 0.0, "fp64_exact_as": 0.0, "fp64": 0.0, "fp64_as": 0.0, "fp64_as1": 0.0, "fp32": 1e-6,
-       "fp32_as": 1e-6, "fp32_as1": 1e-6, "bf16x3": 1e-6, "bf16x3_as": 1e-6, "bf16x3_as1": 1e-6}
+# suffix: none = C-stationary (schedule 0), _asN = A-stationary schedule N
+# (1: one-wave units, 2: four units of a block row per workgroup, 3: narrow
+# sweeps at three waves per SIMD)
+PRECS = {"fp64_exact": sht.PREC_FP64_EXACT, "fp64_exact_as1": sht.PREC_FP64_EXACT, "fp64": sht.PREC_FP64,
+         "fp64_as1": sht.PREC_FP64, "fp64_as2": sht.PREC_FP64, "fp64_as3": sht.PREC_FP64, "fp32": sht.PREC_FP32,
+         "fp32_as1": sht.PREC_FP32, "fp32_as2": sht.PREC_FP32, "fp32_as3": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3,
+         "bf16x3_as1": sht.PREC_BF16X3, "bf16x3_as2": sht.PREC_BF16X3}
+TOL = {k: (0.0 if k.startswith("fp64") else 1e-6) for k in PRECS}
 # v_mfma_f64_16x16x4f64 accumulates its four products as a sequential fused
 # multiply-add chain in k order, so the FP64 MFMA forms equal the fma chain
 # bit for bit (asserted here; the stated contract of SHT_PREC_FP64 is 1e-12).
-EXACT = ("fp64_exact", "fp64_exact_as", "fp64", "fp64_as", "fp64_as1")
+EXACT = tuple(k for k in PRECS if k.startswith("fp64"))
 
 
 def make_plan(ctx, expr, dA, dB, dC, prec):
     plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
-    sched = 1 if prec.endswith("_as") else (2 if prec.endswith("_as1") else 0)
+    sched = int(prec[-1]) if "_as" in prec else 0
     if sched:
         plan.set_schedule(sched)
         assert plan.get_schedule() == sched
@@ -101,7 +101,7 @@ def test_contract_integer_exact_every_form(ctx, expr, ea, eb, ec, prec):
     plan.close()
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as", "fp64", "fp64_as"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as1", "fp64", "fp64_as1", "fp64_as2", "fp64_as3"])
 def test_alpha_beta_kept_pattern(ctx, prec):
     rng = np.random.default_rng(7)
     expr = "ijkl,klm->ijm"
@@ -167,8 +167,8 @@ def test_expression_and_block_size_errors(ctx):
         sht.contract(ctx, "ik,kj->ij", 1.0, A, B3, 0.0, C)
 
 
-@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as", "fp64", "fp64_as", "fp32_as", "bf16x3",
-                                  "bf16x3_as"])
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64_exact_as1", "fp64", "fp64_as1", "fp64_as2", "fp64_as3",
+                                  "fp32_as1", "fp32_as3", "bf16x3", "bf16x3_as1"])
 def test_graph_replay_matches_eager(ctx, prec):
     rng = np.random.default_rng(12)
     expr = "ijk,kl->ijl"
@@ -206,7 +206,7 @@ def test_config0_full_size(ctx):
     dB = sht.Tensor(ctx, bsB)
     dB.reserve_random(seed, 1, 0.3)
     dB.fill_random(seed, 1)
-    for prec in ("fp64_exact", "fp64_exact_as", "fp64", "fp64_as"):
+    for prec in ("fp64_exact", "fp64_exact_as1", "fp64", "fp64_as1", "fp64_as3"):
         dC = sht.Tensor(ctx, bs)
         plan = make_plan(ctx, "ijk,kl->ijl", dA, dB, dC, prec)
         got = plan.tasks()
@@ -233,7 +233,7 @@ def test_autotune_keeps_results(ctx):
     dC = sht.Tensor(ctx, Cp.block_sizes)
     plan = sht.Plan(ctx, expr, dA, dB, dC, sht.PREC_FP64)
     times = plan.autotune(1.0, 0.0, reps=2)
-    assert set(times) == {0, 1, 2} and min(times.values()) > 0 and plan.get_schedule() in (0, 1, 2)
+    assert set(times) == {0, 1, 2, 3} and min(times.values()) > 0 and plan.get_schedule() in (0, 1, 2, 3)
     plan.execute(1.0, 0.0)
     sht.call("sht_device_synchronize")
     assert normwise(dC.download(), R.data) <= 1e-12
