@@ -630,8 +630,7 @@ __device__ inline void run_sweep(const AsArgs& p, const SweepU& sw, const SweepU
 // The records of the next two sweeps and the next sweep's column offsets are
 // loaded one sweep ahead, so a sweep starts without a dependent load chain.
 template <int R, int NSMAX, int F, int WPG, bool A1>
-__global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(NSMAX <= kAsNarrowSlots ? 3 : 1)))
-void k_asp(AsArgs p)
+__global__ __launch_bounds__(64 * WPG) void k_asp(AsArgs p)
 {
   static_assert(16 * NSMAX <= 128, "two column offsets per lane");
   const int wave = WPG == 1 ? 0 : uni(static_cast<int>(threadIdx.x >> 6));
@@ -843,10 +842,10 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
     case AsClass::Bf16x3:
     case AsClass::Bf16x3G: launch_as_bf3(k, a, s); break;
     case AsClass::Mfma64N:
-      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, kAsNarrowSlots, 0, 1, true>), g1, b1, 0, s, a);
-      else hipLaunchKernelGGL((k_asp<2, kAsNarrowSlots, 0, 1, false>), g1, b1, 0, s, a);
+      if (a.alpha_one) hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 0, 1, true>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 0, 1, false>), g1, b1, 0, s, a);
       break;
-    case AsClass::Mfma32N: hipLaunchKernelGGL((k_asp<2, kAsNarrowSlots, 1, 1, false>), g1, b1, 0, s, a); break;
+    case AsClass::Mfma32N: hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 1, 1, false>), g1, b1, 0, s, a); break;
     case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g1, b1, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }

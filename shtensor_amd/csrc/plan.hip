@@ -717,7 +717,7 @@ void build_as_schedule(sht_plan* P)
   };
   std::vector<AsUnit> us = build(brows, exact ? kAsUnitRowsW : kAsUnitRows, 1);
   std::vector<AsUnit> usw = exact ? std::vector<AsUnit>() : build(brows, kAsUnitRows, kAsGroupWaves);
-  std::vector<AsUnit> us4 = narrow ? build(brows4, kAsUnitRows, 1) : std::vector<AsUnit>();
+  std::vector<AsUnit> us4 = narrow ? build(brows4, kAsUnitRowsN, 1) : std::vector<AsUnit>();
   Arena& arena = P->ctx->arena;
   auto upload = [&](const void* src, size_t bytes) {
     void* d = arena.allocate(std::max<size_t>(bytes, 1));

@@ -127,6 +127,7 @@ enum class AsClass : int { Mfma64 = 0, Exact = 1, Mfma64G = 2, Mfma32 = 3, Bf16x
                            Mfma64N = 7, Mfma32N = 8, Count };
 constexpr int kAsUnitRows = 32;    // rows of a unit of the MFMA kernels
 constexpr int kAsUnitRowsW = 64;   // rows of a unit of the fma-chain kernel
+constexpr int kAsUnitRowsN = 64;   // rows of a unit of the narrow schedule (four MFMA tiles)
 constexpr int kAsGroupWaves = 4;   // waves per workgroup of the *G classes
 constexpr int kAsxCols = 8;        // columns per fma-chain slot
 constexpr int kAsxSlots = 8;       // slots per fma-chain sweep
