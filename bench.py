@@ -475,6 +475,12 @@ def main():
                 "peak": round(peak_of(main_f) / 1e12, 1), "unit": "TFLOP/s", "traffic": None}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     roof["intensity_flop_per_byte"] = round(ai, 2)
+    if roof["bound"] == "mfma" and precs[main_f] in (sht.PREC_FP64, sht.PREC_FP64_EXACT):
+        # the FP64 MFMA rate a register-resident loop sustains on this device
+        loop = sht.diag_mfma_f64_rate(stream)
+        roof["mfma_loop_tflops"] = round(loop, 2)
+        roof["frac_of_mfma_loop"] = round(roof["achieved"] / loop, 4)
+        log("FP64 MFMA loop on this device: %.2f TFLOP/s" % loop)
     out = {
         "metric": "contraction GFLOP/s FP64 vs batched/FP32 at 1e-6 rel err; p50/p99 latency",
         "value": round(value, 3),

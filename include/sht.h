@@ -78,6 +78,12 @@ sht_status sht_stream_create(void** stream);
 sht_status sht_stream_destroy(void* stream);
 sht_status sht_stream_synchronize(void* stream);
 
+/* ---- diagnostics (no counterpart in the reference) ------------------------
+ * FP64 MFMA rate (TFLOP/s) of a register-resident loop of independent
+ * v_mfma_f64_16x16x4f64 (two waves per SIMD), run on `stream`: the matrix-core
+ * rate this device sustains, for comparison with the contraction kernels.   */
+sht_status sht_diag_mfma_f64_rate(void* stream, double* tflops);
+
 /* ---- block-product kernel (replaces Kernel<T>, src/Kernel/Kernel.h:37-93,
  *      KernelImpl.cpp:9-73 ctor, 210-770 XMM JIT, 142-208 LAPACK) ----------
  * sht_kernel_create: validates the expression exactly as ContractInfo.h:68-346

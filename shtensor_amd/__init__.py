@@ -434,6 +434,14 @@ class Server:
     __del__ = close
 
 
+def diag_mfma_f64_rate(stream=None):
+    """TFLOP/s of a register-resident FP64 MFMA loop on this device (the
+    matrix-core rate the contraction kernels can be compared with)."""
+    v = C.c_double()
+    call("sht_diag_mfma_f64_rate", _sptr(stream), C.byref(v))
+    return v.value
+
+
 def contract(ctx, expr, alpha, A, B, beta, Cten, precision=PREC_FP64, stream=None):
     """C = beta*C + alpha*contract(expr, A, B) on the GPU (synchronous)."""
     call("sht_contract", C.c_void_p(ctx.ptr), expr.encode(), float(alpha), C.c_void_p(A.ptr), C.c_void_p(B.ptr),

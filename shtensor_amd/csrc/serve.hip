@@ -215,7 +215,9 @@ void finish(sht_server* S, Batch& b)
 
 void worker_loop(sht_server* S)
 {
-  hipSetDevice(S->ctx->device);
+  // (the context's device is already current in the creating thread; a
+  // failure here surfaces as the error of the first batch's HIP calls)
+  (void)hipSetDevice(S->ctx->device);
   for (;;) {
     std::vector<Request*> admitted;
     {
