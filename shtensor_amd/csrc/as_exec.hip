@@ -312,10 +312,11 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // rows are masked (k >= K) and re-read its last row
   const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
   const int nstepsU = (nsteps + AD - 1) / AD * AD;
-  // Every global load below is unconditional (ghost waves included, whose
-  // row chunk is chunk 0 of the block row): a load under a branch makes the
-  // compiler drain all outstanding loads where the paths merge, which would
-  // discard the A and panel prefetch.  Ghosts skip only MFMAs and stores.
+  // Every global load below is unconditional: a load under a branch makes
+  // the compiler drain all outstanding loads where the paths merge, which
+  // would discard the A and panel prefetch.  Ghost waves load their share of
+  // the panel like the others, but their A and C loads all read one fixed
+  // line (rows 0..31 of column 0: L1 hits); they skip MFMAs and stores.
   Acc acc[R][NS];
   if constexpr (F == 0) {
 #pragma unroll
@@ -323,7 +324,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t cb = cbs[colof(q, v)];
-        const double* cq = C + (cb >= 0 ? cb : 0) + rl;
+        const double* cq = C + (cb >= 0 && !ghost ? cb : 0) + rl;
 #pragma unroll
         for (int t = 0; t < R; ++t) acc[t][q][v] = cq[16 * t];
       }
@@ -355,10 +356,10 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   __syncthreads();
   AStream cu, nx;
   cu.a = static_cast<const double*>(p.A) + sw.a_off;
-  cu.K = K;
+  cu.K = ghost ? 1 : K;
   cu.M = M;
   nx.a = static_cast<const double*>(p.A) + nxs.a_off;
-  nx.K = nxs.k;
+  nx.K = ghost ? 1 : nxs.k;
   nx.M = nxs.m;
   TB b[NS];
   auto mma = [&](int st, const double(&araw)[R]) {
@@ -430,7 +431,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
       for (int x = 0; x < G; ++x)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          cb[x][v] = q0 + x < NS ? cbs[colof(q0 + x, v)] : -1;
+          cb[x][v] = q0 + x < NS && !ghost ? cbs[colof(q0 + x, v)] : -1;
           const double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
 #pragma unroll
           for (int t = 0; t < R; ++t) cv[x][v][t] = cq[16 * t];
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
   {
     AStream s;
     s.a = static_cast<const double*>(p.A) + cur.a_off;
-    s.K = cur.k;
+    s.K = ghost ? 1 : cur.k;  // (ghosts read one fixed line)
     s.M = cur.m;
 #pragma unroll
     for (int i = 0; i < AD - 1; ++i) load_a(s, i, r0 + li, lk, ar[i]);
