@@ -37,8 +37,8 @@ HBM_PEAK = 8.0e12
 # kernel class reported by sht_plan_execute_timed -> substring of the kernel's
 # symbol in rocprofv3's output
 KERNEL_SYMBOL = {
-    "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asl<0, 4, true, 8>",
-    "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asl<1, 4, false, 8>", "as_bf16x3": "k_asp<2, 8, 2, 1, false>",
+    "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asl<0, 4, true, 4>",
+    "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asl<1, 4, false, 4>", "as_bf16x3": "k_asp<2, 8, 2, 1, false>",
     "as_bf16x3_g": "k_asp<2, 8, 2, 4, false>", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true>",
     "as_mfma32_n4": "k_asp<4, 4, 1, 1, false>",
     "as_exact": "k_as_exact", "mfma64_n16": "k_mfma64<1>", "mfma64_n32": "k_mfma64<2>",

@@ -337,10 +337,13 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // this wave's share of the panel: slots wave, wave + W, ...; a share past
   // the sweep's slots re-reads slot NS-1 into an unused stage slot
   static_assert(W * ((kAsSweepSlots + W - 1) / W) <= kAsSweepSlots, "stage slots");
-  auto loadS = [&](int st, TB(&g)[NSW]) {
-    const TB* bs = Bl + min(st, last) * bstep;
+  int soff[NSW];
 #pragma unroll
-    for (int x = 0; x < NSW; ++x) g[x] = bs[16 * min(wave + W * x, NS - 1)];
+  for (int x = 0; x < NSW; ++x) soff[x] = 16 * min(wave + W * x, NS - 1);
+  auto loadS = [&](int64_t boff, TB(&g)[NSW]) {
+    const TB* bs = Bl + boff;
+#pragma unroll
+    for (int x = 0; x < NSW; ++x) g[x] = bs[soff[x]];
   };
   auto writeS = [&](int stg, const TB(&g)[NSW]) {
 #pragma unroll
@@ -351,16 +354,25 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   constexpr int BD = 4;
   TB g[BD][NSW];
 #pragma unroll
-  for (int i = 0; i < BD; ++i) loadS(i, g[i]);
+  for (int i = 0; i < BD; ++i) loadS(min(i, last) * bstep, g[i]);
   writeS(0, g[0]);
   __syncthreads();
-  AStream cu, nx;
-  cu.a = static_cast<const double*>(p.A) + sw.a_off;
-  cu.K = ghost ? 1 : K;
-  cu.M = M;
-  nx.a = static_cast<const double*>(p.A) + nxs.a_off;
-  nx.K = ghost ? 1 : nxs.k;
-  nx.M = nxs.m;
+  // panel offset of the next step to load (step s + BD in phase s; stays at
+  // the last row once past it)
+  int64_t bnext = min(BD, last) * bstep;
+  // The A stream, one running offset per lane: the next step to load is
+  // s + AD - 1 in phase s; its column 4*step + lk is clamped to K - 1
+  // (offsets grow with the column, so clamping the offset is the same), and
+  // at step nstepsU the stream moves on to the next sweep's A block.  Ghost
+  // waves use K = 1 (one fixed line).
+  const double* abase = static_cast<const double*>(p.A) + sw.a_off;
+  uint32_t arun = static_cast<uint32_t>((4 * (AD - 1) + lk) * M + rl);
+  uint32_t alast = static_cast<uint32_t>((ghost ? 0 : K - 1) * M + rl);
+  uint32_t a4m = static_cast<uint32_t>(4 * M);
+  const double* nbase = static_cast<const double*>(p.A) + nxs.a_off;
+  const uint32_t nrun = static_cast<uint32_t>(lk * nxs.m + rl);
+  const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + rl);
+  const uint32_t n4m = static_cast<uint32_t>(4 * nxs.m);
   TB b[NS];
   auto mma = [&](int st, const double(&araw)[R]) {
     const bool kin = 4 * st + lk < K;
@@ -383,8 +395,20 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // the other set)
   auto phase = [&](int st, auto jc) {
     constexpr int j = decltype(jc)::value;
-    loadS(st + j + BD, g[j % BD]);  // the set held step st+j, already in LDS
-    load_a2(cu, nx, nstepsU, st + j + AD - 1, rl, lk, ar[(j + AD - 1) % AD]);
+    loadS(bnext, g[j % BD]);  // the set held step st+j, already in LDS
+    bnext += st + j + BD < last ? bstep : 0;
+    if (st + j + AD - 1 == nstepsU) {  // (wave-uniform; no load inside)
+      abase = nbase;
+      arun = nrun;
+      alast = nlast;
+      a4m = n4m;
+    }
+    {
+      const uint32_t off = min(arun, alast);
+#pragma unroll
+      for (int t = 0; t < R; ++t) ar[(j + AD - 1) % AD][t] = __builtin_nontemporal_load(abase + (off + 16u * t));
+      arun += a4m;
+    }
     __builtin_amdgcn_sched_barrier(0);
     if (!ghost) {
 #pragma unroll
@@ -835,11 +859,11 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
       else hipLaunchKernelGGL((k_asp<2, 8, 0, 1, false>), g1, b1, 0, s, a);
       break;
     case AsClass::Mfma64G:
-      if (a.alpha_one) hipLaunchKernelGGL((k_asl<0, G, true, 8>), gg, bg, 0, s, a);
-      else hipLaunchKernelGGL((k_asl<0, G, false, 8>), gg, bg, 0, s, a);
+      if (a.alpha_one) hipLaunchKernelGGL((k_asl<0, G, true, 4>), gg, bg, 0, s, a);
+      else hipLaunchKernelGGL((k_asl<0, G, false, 4>), gg, bg, 0, s, a);
       break;
     case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Mfma32G: hipLaunchKernelGGL((k_asl<1, G, false, 8>), gg, bg, 0, s, a); break;
+    case AsClass::Mfma32G: hipLaunchKernelGGL((k_asl<1, G, false, 4>), gg, bg, 0, s, a); break;
     case AsClass::Bf16x3:
     case AsClass::Bf16x3G: launch_as_bf3(k, a, s); break;
     case AsClass::Mfma64N:
