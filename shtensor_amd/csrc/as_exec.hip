@@ -284,14 +284,17 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
 // The same sweep for a workgroup of W waves that run W row chunks of one
 // block row through the same sweep list: each k-step's panel fragments are
 // fetched once per workgroup (wave w loads the slots q with q % W == w) and
-// shared through LDS (two stages, one workgroup barrier per step), so every
-// wave reads all NS fragments from LDS instead of from L1/L2.  Per step s:
-// issue this wave's panel loads for step s+2, A loads for step s+3, read
-// step s's fragments from LDS, run the MFMAs, write this wave's share of
-// step s+1 (loaded one step earlier) into the other stage, barrier.  A ghost
-// wave (a padding unit of the workgroup) only takes part in the panel loads
-// and barriers.
+// shared through LDS, so every wave reads all NS fragments from LDS instead
+// of from L1/L2.  Per step s: issue this wave's panel loads for step s+4 and
+// A loads for step s+AD-1 (running offsets), read step s's fragments from
+// stage s % 4, run the MFMAs; after every odd step write this wave's share
+// of steps s+1 and s+2 and pass a workgroup barrier.  Stage s % 4 is written
+// before the barrier that precedes step s, and rewritten (with step s+4)
+// only after the barrier that follows step s.  A ghost wave (a padding unit
+// of the workgroup) only takes part in the panel loads and barriers.
 // ---------------------------------------------------------------------------
+constexpr int kAslStages = 4;
+
 template <int NS, int F, bool A1, int W, int AD>
 __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU& nxs, int r0, int li, int lk, int wave,
                                  bool ghost, double (&ar)[AD][2], const int64_t* cbs,
@@ -356,6 +359,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 #pragma unroll
   for (int i = 0; i < BD; ++i) loadS(min(i, last) * bstep, g[i]);
   writeS(0, g[0]);
+  writeS(1, g[1]);
   __syncthreads();
   // panel offset of the next step to load (step s + BD in phase s; stays at
   // the last row once past it)
@@ -390,9 +394,8 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
       }
     }
   };
-  // one step: j = position in the 4-step iteration (ring indices are
-  // compile-time), gs = register set of this step's share (step st+j+1 is in
-  // the other set)
+  // one step s = st + j: j = position in the 4-step iteration, so ring and
+  // stage indexes are compile-time
   auto phase = [&](int st, auto jc) {
     constexpr int j = decltype(jc)::value;
     loadS(bnext, g[j % BD]);  // the set held step st+j, already in LDS
@@ -412,14 +415,17 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
     __builtin_amdgcn_sched_barrier(0);
     if (!ghost) {
 #pragma unroll
-      for (int q = 0; q < NS; ++q) b[q] = stage[j % 2][q][lane];
+      for (int q = 0; q < NS; ++q) b[q] = stage[j][q][lane];
       mma(st + j, ar[j]);
     }
     __builtin_amdgcn_sched_barrier(0);
-    writeS((j + 1) % 2, g[(j + 1) % BD]);
-    __syncthreads();
+    if constexpr (j % 2 == 1) {  // steps s+1, s+2 into the stages of steps s-3, s-2
+      writeS((j + 1) % 4, g[(j + 1) % BD]);
+      writeS((j + 2) % 4, g[(j + 2) % BD]);
+      __syncthreads();
+    }
   };
-  static_assert(AD % BD == 0, "ring sizes");
+  static_assert(AD == 4 && BD == 4 && kAslStages == 4, "stage s % 4 of step s");
   for (int st = 0; st < nstepsU; st += AD) {
     phase(st, std::integral_constant<int, 0>());
     phase(st, std::integral_constant<int, 1>());
@@ -494,7 +500,7 @@ __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
   if (nsw == 0) return;  // a whole-workgroup filler (every wave of it)
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   __shared__ int64_t cbs_all[W][128];
-  __shared__ TB stage[2][kAsSweepSlots][64];
+  __shared__ TB stage[kAslStages][kAsSweepSlots][64];
   int64_t* cbs = cbs_all[wave];
   SweepU cur = to_uni(p.sweeps[s0]);
   SweepU nxt = to_uni(p.sweeps[s0 + min(1, nsw - 1)]);
