@@ -9,7 +9,7 @@ import pytest
 
 import shtensor_amd as sht
 from oracle import oracle as orc
-from gpu_util import make_case, normwise, to_device
+from gpu_util import make_case, normwise, random_bst, to_device
 
 pytestmark = pytest.mark.gpu
 
@@ -238,3 +238,34 @@ def test_autotune_keeps_results(ctx):
     sht.call("sht_device_synchronize")
     assert normwise(dC.download(), R.data) <= 1e-12
     plan.close()
+
+
+@pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (1, 2, 3))])
+def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
+    """Block rows of 1..7 32-row chunks (so the grouped schedule pads its
+    workgroups with 0..3 ghost units) and panels of up to 10 16-column slots
+    (so sweeps are cut at 8 and at 4 slots): every schedule gives the oracle's
+    result, FP64 bit for bit."""
+    rng = np.random.default_rng(77)
+    expr = "ik,kj->ij"
+    bs_i = [5, 32, 33, 64, 97, 130, 161, 200, 224, 17]
+    bs_k = [7, 16, 40]
+    bs_j = [3, 16, 17, 30, 9, 28, 25, 11, 20]
+    A = random_bst(rng, [bs_i, bs_k], 0.8)
+    B = random_bst(rng, [bs_k, bs_j], 0.9)
+    Cp = orc.BlockSparse([bs_i, bs_j])
+    _, _, _, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    outs = {}
+    for sc in scheds:
+        dC = sht.Tensor(ctx, Cp.block_sizes)
+        plan = sht.Plan(ctx, expr, dA, dB, dC, prec)
+        plan.set_schedule(sc)
+        plan.execute(1.0, 0.0)
+        sht.call("sht_device_synchronize")
+        outs[sc] = dC.download()
+        plan.close()
+        if prec == sht.PREC_FP64:
+            assert np.array_equal(outs[sc], R.data), sc
+        else:
+            assert normwise(outs[sc], R.data) <= 1e-6, sc
