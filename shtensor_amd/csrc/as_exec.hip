@@ -10,6 +10,12 @@
 // contributions arrive in the task order (C id, K): no atomics, one fixed
 // order.
 //
+// Kernels: k_asp (one wave per unit; schedule 1, 64-row units for schedule
+// 3, and the split-BF16 groups of schedule 2 sharing panels through L1),
+// k_asl (schedule 2 for FP64/FP32: the four units of a block row in one
+// workgroup, each panel fragment loaded once and shared through LDS),
+// k_as_exact (the fma chain, one row per lane).
+//
 // MFMA operand layouts (lane l of 64; measured on gfx950):
 //   v_mfma_f64_16x16x4f64   A[l%16][l/16], B[l/16][l%16], D[4v + l/16][l%16]
 //   v_mfma_f32_16x16x4f32   A[l%16][l/16], B[l/16][l%16], D[4(l/16) + v][l%16]
