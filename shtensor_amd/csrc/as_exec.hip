@@ -138,6 +138,15 @@ __device__ inline void load_a2(const AStream& cu, const AStream& nx, int nsteps,
   load_a(s, n ? st - nsteps : st, row, lk, a);
 }
 
+// Two consecutive rows of one A column (16 bytes, 8-byte aligned), streamed.
+typedef double v2d8 __attribute__((ext_vector_type(2), aligned(8)));
+__device__ inline __attribute__((unused)) void load_a_pair(const double* a, uint32_t off, double (&x)[2])
+{
+  const v2d8 v = __builtin_nontemporal_load(reinterpret_cast<const v2d8*>(a + off));
+  x[0] = v.x;
+  x[1] = v.y;
+}
+
 template <int F> struct BElem { typedef double type; };
 template <> struct BElem<1> { typedef float type; };
 
@@ -316,7 +325,9 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   const int K = sw.k, M = sw.m;
   double* C = static_cast<double*>(p.C);
   auto colof = [&](int q, int v) { return F == 0 ? 16 * q + 4 * v + lk : 16 * q + 4 * lk + v; };
-  const int rl = r0 + li;
+  // Lane li holds rows r0 + 2li (tile 0) and r0 + 2li + 1 (tile 1): one
+  // 16-byte load covers both tiles' rows of an A or C column.
+  const int rl2 = r0 + 2 * li;
   // AD steps per loop iteration (the A ring's size); steps past the panel's
   // rows are masked (k >= K) and re-read its last row
   const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
@@ -333,9 +344,9 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t cb = cbs[colof(q, v)];
-        const double* cq = C + (cb >= 0 && !ghost ? cb : 0) + rl;
-#pragma unroll
-        for (int t = 0; t < R; ++t) acc[t][q][v] = cq[16 * t];
+        const v2d8 c2 = *reinterpret_cast<const v2d8*>(C + (cb >= 0 && !ghost ? cb : 0) + rl2);
+        acc[0][q][v] = c2.x;
+        acc[1][q][v] = c2.y;
       }
   } else {
 #pragma unroll
@@ -376,12 +387,12 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // at step nstepsU the stream moves on to the next sweep's A block.  Ghost
   // waves use K = 1 (one fixed line).
   const double* abase = static_cast<const double*>(p.A) + sw.a_off;
-  uint32_t arun = static_cast<uint32_t>((4 * (AD - 1) + lk) * M + rl);
-  uint32_t alast = static_cast<uint32_t>((ghost ? 0 : K - 1) * M + rl);
+  uint32_t arun = static_cast<uint32_t>((4 * (AD - 1) + lk) * M + rl2);
+  uint32_t alast = static_cast<uint32_t>((ghost ? 0 : K - 1) * M + rl2);
   uint32_t a4m = static_cast<uint32_t>(4 * M);
   const double* nbase = static_cast<const double*>(p.A) + nxs.a_off;
-  const uint32_t nrun = static_cast<uint32_t>(lk * nxs.m + rl);
-  const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + rl);
+  const uint32_t nrun = static_cast<uint32_t>(lk * nxs.m + rl2);
+  const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + rl2);
   const uint32_t n4m = static_cast<uint32_t>(4 * nxs.m);
   TB b[NS];
   auto mma = [&](int st, const double(&araw)[R]) {
@@ -413,9 +424,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
       a4m = n4m;
     }
     {
-      const uint32_t off = min(arun, alast);
-#pragma unroll
-      for (int t = 0; t < R; ++t) ar[(j + AD - 1) % AD][t] = __builtin_nontemporal_load(abase + (off + 16u * t));
+      load_a_pair(abase, min(arun, alast), ar[(j + AD - 1) % AD]);
       arun += a4m;
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -451,9 +460,14 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t cb = cbs[colof(q, v)];
-#pragma unroll
-        for (int t = 0; t < R; ++t)
-          if (!ghost && cb >= 0 && rl + 16 * t < M) C[cb + rl + 16 * t] = acc[t][q][v];
+        if (!ghost && cb >= 0 && rl2 + 1 < M) {
+          v2d8 c2;
+          c2.x = acc[0][q][v];
+          c2.y = acc[1][q][v];
+          *reinterpret_cast<v2d8*>(C + cb + rl2) = c2;
+        } else if (!ghost && cb >= 0 && rl2 < M) {
+          C[cb + rl2] = acc[0][q][v];
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -468,20 +482,21 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           cb[x][v] = q0 + x < NS && !ghost ? cbs[colof(q0 + x, v)] : -1;
-          const double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
-#pragma unroll
-          for (int t = 0; t < R; ++t) cv[x][v][t] = cq[16 * t];
+          const v2d8 c2 = *reinterpret_cast<const v2d8*>(C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl2);
+          cv[x][v][0] = c2.x;
+          cv[x][v][1] = c2.y;
         }
 #pragma unroll
       for (int x = 0; x < G; ++x) {
         if (q0 + x >= NS) break;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
-#pragma unroll
-          for (int t = 0; t < R; ++t)
-            if (!ghost && cb[x][v] >= 0 && rl + 16 * t < M)
-              cq[16 * t] = cv[x][v][t] + p.alpha * static_cast<double>(acc[t][q0 + x][v]);
+          double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl2;
+          v2d8 c2;
+          c2.x = cv[x][v][0] + p.alpha * static_cast<double>(acc[0][q0 + x][v]);
+          c2.y = cv[x][v][1] + p.alpha * static_cast<double>(acc[1][q0 + x][v]);
+          if (!ghost && cb[x][v] >= 0 && rl2 + 1 < M) *reinterpret_cast<v2d8*>(cq) = c2;
+          else if (!ghost && cb[x][v] >= 0 && rl2 < M) cq[0] = c2.x;
         }
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -515,12 +530,12 @@ __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
   load_cols(p, cur, lane, cc0, cc1);
   double ar[AD][2];
   {
-    AStream s;
-    s.a = static_cast<const double*>(p.A) + cur.a_off;
-    s.K = ghost ? 1 : cur.k;  // (ghosts read one fixed line)
-    s.M = cur.m;
+    // first steps of the A stream (rows r0 + 2li, +1; ghosts: one fixed line)
+    const double* a = static_cast<const double*>(p.A) + cur.a_off;
+    const int kmax = ghost ? 0 : cur.k - 1;
 #pragma unroll
-    for (int i = 0; i < AD - 1; ++i) load_a(s, i, r0 + li, lk, ar[i]);
+    for (int i = 0; i < AD - 1; ++i)
+      load_a_pair(a, static_cast<uint32_t>(min(4 * i + lk, kmax) * cur.m + r0 + 2 * li), ar[i]);
   }
   for (int j = 0; j < nsw; ++j) {
     cbs[lane] = cc0;
