@@ -627,46 +627,46 @@ void build_as_schedule(sht_plan* P)
     double work;  // sum over sweeps of k * columns
   };
   auto gen_sweeps = [&](int cap, std::vector<AsSweep>& sweeps, std::vector<int64_t>& ccol, std::vector<Row>& brows) {
-  std::vector<int64_t> pcof;  // partner C offsets of the current A block
-  int64_t t = 0;
-  while (t < nt) {
-    const int64_t I = aI[ta[order[t]]];
-    Row row{static_cast<int32_t>(sweeps.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
-    while (t < nt && aI[ta[order[t]]] == I) {
-      const int64_t a = ta[order[t]];
-      const Panel& pn = panels[panel_of[tb[order[t]]]];
-      pcof.assign(pn.npart, -1);
-      while (t < nt && ta[order[t]] == a) {
-        pcof[part_of[tb[order[t]]]] = C->offsets[tc[order[t]]];
-        ++t;
-      }
-      const int32_t k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
-      const int nslots = (pn.ncol + kBtPad - 1) / kBtPad;
-      for (int g = 0; g < nslots; g += cap) {
-        AsSweep sw{};
-        sw.a_off = A->offsets[a];
-        sw.b_off = bf3 ? pn.bt_off + static_cast<int64_t>(16 * g) * pn.pitch : pn.bt_off + 16 * g;
-        sw.c0 = static_cast<int64_t>(ccol.size());
-        sw.k = k;
-        sw.m = static_cast<int32_t>(row.rows);
-        sw.pitch = pn.pitch;
-        sw.ns = std::min(cap, nslots - g);
-        for (int c = 16 * g; c < 16 * (g + sw.ns); ++c) {
-          int64_t v = -1;
-          if (c < pn.ncol) {
-            const AsCol& cm = cols[pn.col0 + c];
-            if (pcof[cm.p] >= 0) v = pcof[cm.p] + static_cast<int64_t>(cm.j) * row.rows;
-          }
-          ccol.push_back(v);
+    std::vector<int64_t> pcof;  // partner C offsets of the current A block
+    int64_t t = 0;
+    while (t < nt) {
+      const int64_t I = aI[ta[order[t]]];
+      Row row{static_cast<int32_t>(sweeps.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
+      while (t < nt && aI[ta[order[t]]] == I) {
+        const int64_t a = ta[order[t]];
+        const Panel& pn = panels[panel_of[tb[order[t]]]];
+        pcof.assign(pn.npart, -1);
+        while (t < nt && ta[order[t]] == a) {
+          pcof[part_of[tb[order[t]]]] = C->offsets[tc[order[t]]];
+          ++t;
         }
-        row.work += static_cast<double>(k) * 16 * sw.ns;
-        sweeps.push_back(sw);
+        const int32_t k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
+        const int nslots = (pn.ncol + kBtPad - 1) / kBtPad;
+        for (int g = 0; g < nslots; g += cap) {
+          AsSweep sw{};
+          sw.a_off = A->offsets[a];
+          sw.b_off = bf3 ? pn.bt_off + static_cast<int64_t>(16 * g) * pn.pitch : pn.bt_off + 16 * g;
+          sw.c0 = static_cast<int64_t>(ccol.size());
+          sw.k = k;
+          sw.m = static_cast<int32_t>(row.rows);
+          sw.pitch = pn.pitch;
+          sw.ns = std::min(cap, nslots - g);
+          for (int c = 16 * g; c < 16 * (g + sw.ns); ++c) {
+            int64_t v = -1;
+            if (c < pn.ncol) {
+              const AsCol& cm = cols[pn.col0 + c];
+              if (pcof[cm.p] >= 0) v = pcof[cm.p] + static_cast<int64_t>(cm.j) * row.rows;
+            }
+            ccol.push_back(v);
+          }
+          row.work += static_cast<double>(k) * 16 * sw.ns;
+          sweeps.push_back(sw);
+        }
       }
+      row.nsweep = static_cast<int32_t>(sweeps.size()) - row.sweep0;
+      brows.push_back(row);
     }
-    row.nsweep = static_cast<int32_t>(sweeps.size()) - row.sweep0;
-    brows.push_back(row);
-  }
-  SHT_REQUIRE(sweeps.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
+    SHT_REQUIRE(sweeps.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
   };
   std::vector<AsSweep> sweeps, sweeps4;
   std::vector<int64_t> ccol, ccol4;
