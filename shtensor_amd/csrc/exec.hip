@@ -38,6 +38,7 @@ __global__ __launch_bounds__(64) void k_exact(ExecArgs p)
   constexpr int KC = 32;
   __shared__ double Bs[KC * NC];
   const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
   const int M = cb.m, N = cb.n, n0 = tile.n0;
   const int nc = min(NC, N - n0);
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 {
   constexpr int RT = 4;
   const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
   const int M = cb.m, N = cb.n;
   const int lane = threadIdx.x;
@@ -176,6 +178,7 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
 {
   constexpr int RT = 4;
   const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
   const int M = cb.m, N = cb.n;
   const int lane = threadIdx.x;
@@ -275,6 +278,7 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
 {
   constexpr int RT = 4;
   const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
   const int M = cb.m, N = cb.n;
   const int lane = threadIdx.x;

@@ -460,10 +460,28 @@ void build_schedule(sht_plan* P)
                       static_cast<double>(P->C->nelem));
   P->bytes_c = es * static_cast<double>(P->C->nelem);
 
-  // tiles per kernel class, each class ordered by descending cost
+  // Tiles per kernel class.  When A is the larger operand (at least twice
+  // B), an XCD-aware order: workgroup g runs on XCD g % 8 and every XCD has
+  // its own L2, so the tiles of one C block row (which read the same A
+  // blocks) go to one XCD, consecutively, and those blocks are shared in that
+  // L2.  Rows are dealt to the XCD with the least accumulated cost (most
+  // expensive first), the eight queues are interleaved, and short queues are
+  // padded with no-work tiles.  Otherwise (A and B alike, where the C block
+  // order already keeps tiles that share B blocks together) tiles are ordered
+  // by descending cost.
   struct T {
     TileRec r;
     double cost;
+    int64_t row;  // rolled block index of C's row (free-A) axes
+  };
+  const bool by_row = P->A->nelem >= 2 * P->B->nelem;
+  auto row_key = [&](int64_t id) {
+    int64_t k = 0, mul = 1;
+    for (int ax : m.c_row) {
+      k += ((id / P->C->bstride[ax]) % P->C->nblk[ax]) * mul;
+      mul *= P->C->nblk[ax];
+    }
+    return k;
   };
   std::vector<std::vector<T>> per(static_cast<int>(KClass::Count));
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
@@ -475,20 +493,54 @@ void build_schedule(sht_plan* P)
     else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : KClass::Bf16x32;
     else k = N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
     const int rows = kclass_rows(k), cols = kclass_cols(k);
+    const int64_t rk = row_key(P->C->indices[c]);
     for (int m0 = 0; m0 < M; m0 += rows)
       for (int n0 = 0; n0 < N; n0 += cols) {
         const double cost = static_cast<double>(std::min(rows, M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
-        per[static_cast<int>(k)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost});
+        per[static_cast<int>(k)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost, rk});
       }
   }
+  constexpr int kXcd = 8;
   std::vector<TileRec> all;
   P->launches.clear();
   for (int k = 0; k < static_cast<int>(KClass::Count); ++k) {
     auto& v = per[k];
     if (v.empty()) continue;
-    std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
-    P->launches.push_back({static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size())});
-    for (const T& t : v) all.push_back(t.r);
+    if (!by_row) {
+      std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
+      P->launches.push_back({static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size())});
+      for (const T& t : v) all.push_back(t.r);
+      continue;
+    }
+    // rows: tiles grouped by row key, rows by descending total cost
+    std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.row < y.row; });
+    struct RowSpan {
+      size_t b, e;
+      double cost;
+    };
+    std::vector<RowSpan> rows;
+    for (size_t i = 0; i < v.size();) {
+      size_t e = i;
+      double c = 0.0;
+      while (e < v.size() && v[e].row == v[i].row) c += v[e++].cost;
+      rows.push_back({i, e, c});
+      i = e;
+    }
+    std::stable_sort(rows.begin(), rows.end(), [](const RowSpan& x, const RowSpan& y) { return x.cost > y.cost; });
+    std::vector<std::vector<TileRec>> q(kXcd);
+    std::vector<double> load(kXcd, 0.0);
+    for (const RowSpan& r : rows) {
+      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      for (size_t i = r.b; i < r.e; ++i) q[x].push_back(v[i].r);
+      load[x] += r.cost;
+    }
+    size_t longest = 0;
+    for (const auto& qq : q) longest = std::max(longest, qq.size());
+    const int64_t first = static_cast<int64_t>(all.size());
+    for (size_t slot = 0; slot < longest; ++slot)
+      for (int x = 0; x < kXcd; ++x) all.push_back(slot < q[x].size() ? q[x][slot] : TileRec{0, 0, 0, 1});
+    while (!all.empty() && static_cast<int64_t>(all.size()) > first && all.back().pad) all.pop_back();
+    P->launches.push_back({static_cast<KClass>(k), first, static_cast<int64_t>(all.size()) - first});
   }
   if (!all.empty()) {
     P->d_tiles = static_cast<TileRec*>(arena.allocate(all.size() * sizeof(TileRec)));

@@ -9,7 +9,8 @@
 namespace sht {
 
 // One C-stationary tile: rows [m0, m0+rows) and columns [n0, n0+cols) of
-// C block `cb`; the kernel class fixes rows/cols.
+// C block `cb`; the kernel class fixes rows/cols.  pad = 1: a padding tile
+// (no work) that keeps the XCD-interleaved tile order aligned.
 struct TileRec {
   int32_t cb;
   int32_t m0;
