@@ -103,6 +103,10 @@ __global__ __launch_bounds__(64) void k_exact(ExecArgs p)
 //   A frag: lane l holds A[l%16][l/16]; B frag: B[l/16][l%16];
 //   D: lane l, item v holds D[4*v + l/16][l%16]  (the f32 form below differs:
 //   D[4*(l/16) + v][l%16]).
+// The kernels compute the transposed tile, D = (B fragment)^T x (A
+// fragment)^T (products commute: every element is the same chain), so lane
+// l holds row l%16 of four C columns and each C load and store instruction
+// covers 16 consecutive rows of four columns.
 // ---------------------------------------------------------------------------
 template <int CT>
 __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
@@ -124,8 +128,8 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * v + lk;
-        const int j = tile.n0 + 16 * ct + li;
+        const int i = tile.m0 + 16 * rt + li;
+        const int j = tile.n0 + 16 * ct + 4 * v + lk;
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero) {
@@ -153,7 +157,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int ct = 0; ct < CT; ++ct)
-            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
       }
     }
   }
@@ -163,8 +167,8 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * v + lk;
-        const int j = tile.n0 + 16 * ct + li;
+        const int i = tile.m0 + 16 * rt + li;
+        const int j = tile.n0 + 16 * ct + 4 * v + lk;
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
 }
@@ -193,8 +197,8 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * lk + v;
-        const int j = tile.n0 + 16 * ct + li;
+        const int i = tile.m0 + 16 * rt + li;
+        const int j = tile.n0 + 16 * ct + 4 * lk + v;
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero) {
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int ct = 0; ct < CT; ++ct)
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[ct], a[rt], part[rt][ct], 0, 0, 0);
       }
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
@@ -242,8 +246,8 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * lk + v;
-        const int j = tile.n0 + 16 * ct + li;
+        const int i = tile.m0 + 16 * rt + li;
+        const int j = tile.n0 + 16 * ct + 4 * lk + v;
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
 }
@@ -293,8 +297,8 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * lk + v;
-        const int j = tile.n0 + 16 * ct + li;
+        const int i = tile.m0 + 16 * rt + li;
+        const int j = tile.n0 + 16 * ct + 4 * lk + v;
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero) {
@@ -338,12 +342,12 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
           }
 #pragma unroll
           for (int ct = 0; ct < CT; ++ct) {
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ct], part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[ct], part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ct], part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[ct], part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[ct], part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ct], part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], al, part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm[ct], am, part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[ct], ah, part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], am, part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm[ct], ah, part[rt][ct], 0, 0, 0);
+            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], ah, part[rt][ct], 0, 0, 0);
           }
         }
       }
@@ -361,8 +365,8 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
     for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + 4 * lk + v;
-        const int j = tile.n0 + 16 * ct + li;
+        const int i = tile.m0 + 16 * rt + li;
+        const int j = tile.n0 + 16 * ct + 4 * lk + v;
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
 }
