@@ -117,3 +117,11 @@ def test_kernel_errors_and_info():
     assert "Expression: jik, kmj -> im" in info
     assert "map_row: 1\nmap_col: 0,2\n" in info
     assert "scatter row: 0,6,12,18,24\n" in info
+
+
+def test_diag_mfma_rate():
+    """The FP64 MFMA loop probe runs and reports a rate in a plausible range
+    (well above zero, not above the matrix cores' 78.6 TFLOP/s peak by more
+    than clock slack)."""
+    r = sht.diag_mfma_f64_rate()
+    assert 5.0 < r < 90.0, r
