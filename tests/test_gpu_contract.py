@@ -243,14 +243,15 @@ def test_autotune_keeps_results(ctx):
 @pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (1, 2, 3))])
 def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
     """Block rows of 1..7 32-row chunks (so the grouped schedule pads its
-    workgroups with 0..3 ghost units) and panels of up to 10 16-column slots
-    (so sweeps are cut at 8 and at 4 slots): every schedule gives the oracle's
-    result, FP64 bit for bit."""
+    workgroups with 0..3 ghost units), panels of up to 17 16-column slots
+    (so sweeps are cut at 8 and at 4 slots) and C blocks up to 70 columns wide
+    (C-stationary 64-column tiles plus a remainder): every schedule gives the
+    oracle's result, FP64 bit for bit."""
     rng = np.random.default_rng(77)
     expr = "ik,kj->ij"
     bs_i = [5, 32, 33, 64, 97, 130, 161, 200, 224, 17]
     bs_k = [7, 16, 40]
-    bs_j = [3, 16, 17, 30, 9, 28, 25, 11, 20]
+    bs_j = [3, 16, 17, 30, 9, 28, 25, 11, 20, 40, 70]
     A = random_bst(rng, [bs_i, bs_k], 0.8)
     B = random_bst(rng, [bs_k, bs_j], 0.9)
     Cp = orc.BlockSparse([bs_i, bs_j])
