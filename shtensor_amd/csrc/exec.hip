@@ -108,6 +108,14 @@ __global__ __launch_bounds__(64) void k_exact(ExecArgs p)
 // l holds row l%16 of four C columns and each C load and store instruction
 // covers 16 consecutive rows of four columns.
 // ---------------------------------------------------------------------------
+// Orders this wave's LDS accesses (each wave owns its LDS buffer).
+__device__ inline void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int CT>
 __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 {
@@ -161,16 +169,26 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
       }
     }
   }
+  // Write-back four columns at a time through LDS: lane (li, lk) holds rows
+  // 16rt + li of column 4v + lk of the group; lane l then stores row m0 + l
+  // of each column, so every store instruction writes 64 consecutive rows
+  // (512 bytes) of one column.
+  __shared__ double cst[4 * 64];
+  const int i = tile.m0 + lane;
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
+  for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
+    for (int v = 0; v < 4; ++v) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + li;
-        const int j = tile.n0 + 16 * ct + 4 * v + lk;
-        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
+      for (int rt = 0; rt < RT; ++rt) cst[lk * 64 + 16 * rt + li] = acc[rt][ct][v];
+      wave_lds_sync();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = tile.n0 + 16 * ct + 4 * v + c;
+        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[c * 64 + lane];
       }
+      wave_lds_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
