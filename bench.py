@@ -41,10 +41,12 @@ KERNEL_SYMBOL = {
     "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asl<1, 4, false, 4>", "as_bf16x3": "k_asp<2, 8, 2, 1, false>",
     "as_bf16x3_g": "k_asp<2, 8, 2, 4, false>", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true>",
     "as_mfma32_n4": "k_asp<4, 4, 1, 1, false>",
-    "as_exact": "k_as_exact", "mfma64_n16": "k_mfma64<1>", "mfma64_n32": "k_mfma64<2>",
-    "mfma32_n16": "k_mfma32<1>", "mfma32_n32": "k_mfma32<2>", "bf16x3_n16": "k_bf16x3<1>",
-    "bf16x3_n32": "k_bf16x3<2>", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
+    "as_exact": "k_as_exact", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
 }
+for _name, _kern in (("mfma64", "k_mfma64"), ("mfma32", "k_mfma32"), ("bf16x3", "k_bf16x3")):
+    for _ct, _n in ((1, 16), (2, 32)):
+        KERNEL_SYMBOL["%s_n%d" % (_name, _n)] = "%s<%d, 4>" % (_kern, _ct)
+        KERNEL_SYMBOL["%s_n%d_r32" % (_name, _n)] = "%s<%d, 2>" % (_kern, _ct)
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
 
 CONFIGS = {

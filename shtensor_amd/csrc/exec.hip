@@ -9,12 +9,16 @@
 //    (src/Kernel/KernelImpl.cpp:594-710) continued across tasks: bitwise equal
 //    to the CPU chain.  One wave, 2 rows per lane (128 rows), NC columns held
 //    in registers; B chunks staged in LDS and read as wave-wide broadcasts.
-//  * k_mfma64<CT> — v_mfma_f64_16x16x4f64, 64 rows x 16*CT columns per wave,
-//    accumulators persist across tasks (FP64 throughout).
-//  * k_mfma32<CT> — v_mfma_f32_16x16x4f32 per task, task partial added into
-//    an FP64 accumulator (FP64 across blocks).
-//  * k_bf16x3<CT> — three-term split-BF16 (x = hi + mid + lo), six products
-//    of order <= 2 with v_mfma_f32_16x16x32_bf16, FP32 per task, FP64 across.
+//  * k_mfma64<CT,RT> — v_mfma_f64_16x16x4f64, 16*RT rows x 16*CT columns per
+//    wave, accumulators persist across tasks (FP64 throughout).
+//  * k_mfma32<CT,RT> — v_mfma_f32_16x16x4f32 per task, task partial added
+//    into an FP64 accumulator (FP64 across blocks).
+//  * k_bf16x3<CT,RT> — three-term split-BF16 (x = hi + mid + lo), six
+//    products of order <= 2 with v_mfma_f32_16x16x32_bf16, FP32 per task,
+//    FP64 across.
+//  RT = 4 (64-row tiles) in general; RT = 2 (32-row tiles) for a block's
+//  last row tile when at most 32 rows remain, so blocks of <= 32 rows (and
+//  the remainders of larger ones) issue no MFMAs on padding rows.
 
 #include "plan.hpp"
 
@@ -116,10 +120,9 @@ __device__ inline void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int CT>
+template <int CT, int RT>
 __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 {
-  constexpr int RT = 4;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
@@ -170,22 +173,24 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
     }
   }
   // Write-back four columns at a time through LDS: lane (li, lk) holds rows
-  // 16rt + li of column 4v + lk of the group; lane l then stores row m0 + l
-  // of each column, so every store instruction writes 64 consecutive rows
-  // (512 bytes) of one column.
-  __shared__ double cst[4 * 64];
-  const int i = tile.m0 + lane;
+  // 16rt + li of column 4v + lk of the group; lane l then stores row
+  // m0 + l % R of column l / R of each group of 64 / R columns, so every
+  // store instruction writes whole R-row column pieces (64 x 8 bytes).
+  constexpr int R = 16 * RT;
+  __shared__ double cst[4 * R];
+  const int i = tile.m0 + lane % R;
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) cst[lk * 64 + 16 * rt + li] = acc[rt][ct][v];
+      for (int rt = 0; rt < RT; ++rt) cst[lk * R + 16 * rt + li] = acc[rt][ct][v];
       wave_lds_sync();
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int j = tile.n0 + 16 * ct + 4 * v + c;
-        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[c * 64 + lane];
+      for (int c = 0; c < 4; c += 64 / R) {
+        const int cc = c + lane / R;
+        const int j = tile.n0 + 16 * ct + 4 * v + cc;
+        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[cc * R + lane % R];
       }
       wave_lds_sync();
     }
@@ -195,10 +200,9 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 // FP32 MFMA per task, FP64 across tasks.  v_mfma_f32_16x16x4f32 has the same
 // operand/result layout as the f64 variant above.
 // ---------------------------------------------------------------------------
-template <int CT>
+template <int CT, int RT>
 __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
 {
-  constexpr int RT = 4;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
@@ -295,10 +299,9 @@ __device__ inline void split3_cs(double xd, short& h, short& m, short& l)
   l = static_cast<short>(bf16_bits_cs(r2));
 }
 
-template <int CT>
+template <int CT, int RT>
 __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
 {
-  constexpr int RT = 4;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // a padding tile of the XCD-interleaved order
   const CBlkRec cb = p.cblk[tile.cb];
@@ -435,6 +438,12 @@ int kclass_rows(KClass k)
     case KClass::Exact8:
     case KClass::Exact16:
     case KClass::Exact32: return 128;
+    case KClass::Mfma16h:
+    case KClass::Mfma32h:
+    case KClass::F32x16h:
+    case KClass::F32x32h:
+    case KClass::Bf16x16h:
+    case KClass::Bf16x32h: return 32;
     default: return 64;
   }
 }
@@ -447,7 +456,10 @@ int kclass_cols(KClass k)
     case KClass::Exact32: return 32;
     case KClass::Mfma16:
     case KClass::F32x16:
-    case KClass::Bf16x16: return 16;
+    case KClass::Bf16x16:
+    case KClass::Mfma16h:
+    case KClass::F32x16h:
+    case KClass::Bf16x16h: return 16;
     default: return 32;
   }
 }
@@ -460,12 +472,18 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Exact8: hipLaunchKernelGGL(k_exact<8>, g, b, 0, s, a); break;
     case KClass::Exact16: hipLaunchKernelGGL(k_exact<16>, g, b, 0, s, a); break;
     case KClass::Exact32: hipLaunchKernelGGL(k_exact<32>, g, b, 0, s, a); break;
-    case KClass::Mfma16: hipLaunchKernelGGL(k_mfma64<1>, g, b, 0, s, a); break;
-    case KClass::Mfma32: hipLaunchKernelGGL(k_mfma64<2>, g, b, 0, s, a); break;
-    case KClass::F32x16: hipLaunchKernelGGL(k_mfma32<1>, g, b, 0, s, a); break;
-    case KClass::F32x32: hipLaunchKernelGGL(k_mfma32<2>, g, b, 0, s, a); break;
-    case KClass::Bf16x16: hipLaunchKernelGGL(k_bf16x3<1>, g, b, 0, s, a); break;
-    case KClass::Bf16x32: hipLaunchKernelGGL(k_bf16x3<2>, g, b, 0, s, a); break;
+    case KClass::Mfma16: hipLaunchKernelGGL((k_mfma64<1, 4>), g, b, 0, s, a); break;
+    case KClass::Mfma32: hipLaunchKernelGGL((k_mfma64<2, 4>), g, b, 0, s, a); break;
+    case KClass::F32x16: hipLaunchKernelGGL((k_mfma32<1, 4>), g, b, 0, s, a); break;
+    case KClass::F32x32: hipLaunchKernelGGL((k_mfma32<2, 4>), g, b, 0, s, a); break;
+    case KClass::Bf16x16: hipLaunchKernelGGL((k_bf16x3<1, 4>), g, b, 0, s, a); break;
+    case KClass::Bf16x32: hipLaunchKernelGGL((k_bf16x3<2, 4>), g, b, 0, s, a); break;
+    case KClass::Mfma16h: hipLaunchKernelGGL((k_mfma64<1, 2>), g, b, 0, s, a); break;
+    case KClass::Mfma32h: hipLaunchKernelGGL((k_mfma64<2, 2>), g, b, 0, s, a); break;
+    case KClass::F32x16h: hipLaunchKernelGGL((k_mfma32<1, 2>), g, b, 0, s, a); break;
+    case KClass::F32x32h: hipLaunchKernelGGL((k_mfma32<2, 2>), g, b, 0, s, a); break;
+    case KClass::Bf16x16h: hipLaunchKernelGGL((k_bf16x3<1, 2>), g, b, 0, s, a); break;
+    case KClass::Bf16x32h: hipLaunchKernelGGL((k_bf16x3<2, 2>), g, b, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
   SHT_HIP(hipGetLastError());

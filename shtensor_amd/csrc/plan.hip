@@ -426,6 +426,20 @@ void build_tasks(sht_plan* P)
   SHT_HIP(hipStreamSynchronize(s));
 }
 
+// The 32-row form of a 64-row MFMA tile class.
+KClass half_rows(KClass k)
+{
+  switch (k) {
+    case KClass::Mfma16: return KClass::Mfma16h;
+    case KClass::Mfma32: return KClass::Mfma32h;
+    case KClass::F32x16: return KClass::F32x16h;
+    case KClass::F32x32: return KClass::F32x32h;
+    case KClass::Bf16x16: return KClass::Bf16x16h;
+    case KClass::Bf16x32: return KClass::Bf16x32h;
+    default: return k;
+  }
+}
+
 void build_schedule(sht_plan* P)
 {
   Arena& arena = P->ctx->arena;
@@ -494,11 +508,15 @@ void build_schedule(sht_plan* P)
     else k = N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
     const int rows = kclass_rows(k), cols = kclass_cols(k);
     const int64_t rk = row_key(P->C->indices[c]);
-    for (int m0 = 0; m0 < M; m0 += rows)
+    for (int m0 = 0; m0 < M; m0 += rows) {
+      // the last row tile of an MFMA class takes the 32-row form when it can
+      const KClass kt = (!exact && M - m0 <= 32) ? half_rows(k) : k;
       for (int n0 = 0; n0 < N; n0 += cols) {
-        const double cost = static_cast<double>(std::min(rows, M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
-        per[static_cast<int>(k)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost, rk});
+        const double cost =
+            static_cast<double>(std::min(kclass_rows(kt), M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
+        per[static_cast<int>(kt)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost, rk});
       }
+    }
   }
   constexpr int kXcd = 8;
   std::vector<TileRec> all;
@@ -1038,8 +1056,10 @@ const char* sht_kclass_name(int k)
   static const char* as_names[] = {"as_mfma64",   "as_exact",    "as_mfma64_g",  "as_mfma32",   "as_bf16x3",
                                    "as_mfma32_g", "as_bf16x3_g", "as_mfma64_n4", "as_mfma32_n4"};
   if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
-  static const char* names[] = {"exact_n8", "exact_n16", "exact_n32", "mfma64_n16", "mfma64_n32",
-                                "mfma32_n16", "mfma32_n32", "bf16x3_n16", "bf16x3_n32"};
+  static const char* names[] = {"exact_n8",      "exact_n16",     "exact_n32",     "mfma64_n16",
+                                "mfma64_n32",    "mfma32_n16",    "mfma32_n32",    "bf16x3_n16",
+                                "bf16x3_n32",    "mfma64_n16_r32", "mfma64_n32_r32", "mfma32_n16_r32",
+                                "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
   return names[k];
 }

@@ -51,6 +51,9 @@ enum class KClass : int {
   Mfma16, Mfma32,                    // FP64 MFMA, 64 rows x {16,32} cols
   F32x16, F32x32,                    // FP32 MFMA, FP64 cross-task
   Bf16x16, Bf16x32,                  // split-BF16x3 MFMA, FP64 cross-task
+  Mfma16h, Mfma32h,                  // the three MFMA forms above on 32-row
+  F32x16h, F32x32h,                  // tiles (a block's last row tile when
+  Bf16x16h, Bf16x32h,                // at most 32 rows remain)
   Count
 };
 
