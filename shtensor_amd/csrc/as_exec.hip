@@ -96,6 +96,22 @@ __device__ inline void load_cols(const AsArgs& p, const SweepU& sw, int lane, in
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
+// A sweep column entry: -1 (pad column or dropped block), an element offset
+// v of C(0, column), or -(v + 2) for the first sweep of the block row that
+// touches the column.
+__device__ inline int64_t ccol_off(int64_t e) { return e < -1 ? -e - 2 : e; }
+
+// Where the values a sweep column starts from are read, for rows from
+// `row`: the C column; for a column's first sweep with p.zero set (beta ==
+// 0), a line of zeros (C is not read); for a pad column, column 0 of C (its
+// values are never stored).  Branch-free (address selects only).
+__device__ inline const double* c_src(const AsArgs& p, int64_t e, int row)
+{
+  const double* C = static_cast<const double*>(p.C);
+  const double* zs = p.zero ? p.zero + (row & 63) : C + (-e - 2) + row;
+  return e >= 0 ? C + e + row : (e == -1 ? C + row : zs);
+}
+
 // ---------------------------------------------------------------------------
 // The A operand of a unit is one stream across all its sweeps: a ring of
 // four register sets holds the A fragments of the next steps, filled three
@@ -183,8 +199,7 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
     for (int q = 0; q < NS; ++q)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t cb = cbs[colof(q, v)];
-        const double* cq = C + (cb >= 0 ? cb : 0) + rl;
+        const double* cq = c_src(p, cbs[colof(q, v)], rl);
 #pragma unroll
         for (int t = 0; t < R; ++t) acc[t][q][v] = cq[16 * t];
       }
@@ -254,7 +269,7 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t cb = cbs[colof(q, v)];
+        const int64_t cb = ccol_off(cbs[colof(q, v)]);
 #pragma unroll
         for (int t = 0; t < R; ++t)
           if (cb >= 0 && rl + 16 * t < M) C[cb + rl + 16 * t] = acc[t][q][v];
@@ -273,8 +288,9 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
       for (int x = 0; x < G; ++x)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          cb[x][v] = q0 + x < NS ? cbs[colof(q0 + x, v)] : -1;
-          const double* cq = C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl;
+          const int64_t e = q0 + x < NS ? cbs[colof(q0 + x, v)] : -1;
+          cb[x][v] = ccol_off(e);
+          const double* cq = c_src(p, e, rl);
 #pragma unroll
           for (int t = 0; t < R; ++t) cv[x][v][t] = cq[16 * t];
         }
@@ -343,8 +359,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
     for (int q = 0; q < NS; ++q)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t cb = cbs[colof(q, v)];
-        const v2d8 c2 = *reinterpret_cast<const v2d8*>(C + (cb >= 0 && !ghost ? cb : 0) + rl2);
+        const v2d8 c2 = *reinterpret_cast<const v2d8*>(ghost ? C + rl2 : c_src(p, cbs[colof(q, v)], rl2));
         acc[0][q][v] = c2.x;
         acc[1][q][v] = c2.y;
       }
@@ -459,7 +474,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t cb = cbs[colof(q, v)];
+        const int64_t cb = ccol_off(cbs[colof(q, v)]);
         if (!ghost && cb >= 0 && rl2 + 1 < M) {
           v2d8 c2;
           c2.x = acc[0][q][v];
@@ -481,8 +496,9 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
       for (int x = 0; x < G; ++x)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          cb[x][v] = q0 + x < NS && !ghost ? cbs[colof(q0 + x, v)] : -1;
-          const v2d8 c2 = *reinterpret_cast<const v2d8*>(C + (cb[x][v] >= 0 ? cb[x][v] : 0) + rl2);
+          const int64_t e = q0 + x < NS && !ghost ? cbs[colof(q0 + x, v)] : -1;
+          cb[x][v] = ccol_off(e);
+          const v2d8 c2 = *reinterpret_cast<const v2d8*>(c_src(p, e, rl2));
           cv[x][v][0] = c2.x;
           cv[x][v][1] = c2.y;
         }
@@ -652,8 +668,9 @@ __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int 
     int64_t cb[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      cb[v] = cbs[16 * q + 4 * lk + v];
-      const double* cq = C + (cb[v] >= 0 ? cb[v] : 0) + rl;
+      const int64_t e = cbs[16 * q + 4 * lk + v];
+      cb[v] = ccol_off(e);
+      const double* cq = c_src(p, e, rl);
 #pragma unroll
       for (int t = 0; t < R; ++t) cv[v][t] = cq[16 * t];
     }
@@ -762,8 +779,8 @@ __global__ __launch_bounds__(64) void k_as_exact(AsArgs p)
       for (int q = 0; q < S; ++q)
 #pragma unroll
         for (int x = 0; x < W; ++x) {
-          const int64_t cb = q < ns ? p.ccol[sw.c0 + c0 + q * W + x] : -1;
-          acc[q][x] = (cb >= 0 && rowok) ? C[cb + row] : 0.0;
+          const int64_t e = q < ns ? p.ccol[sw.c0 + c0 + q * W + x] : -1;
+          acc[q][x] = (e != -1 && rowok) ? *c_src(p, e, row) : 0.0;
         }
       const double* bt = BT + sw.b_off + c0;
       for (int k = 0; k < K; ++k) {
@@ -782,7 +799,7 @@ __global__ __launch_bounds__(64) void k_as_exact(AsArgs p)
       for (int q = 0; q < S; ++q)
 #pragma unroll
         for (int x = 0; x < W; ++x) {
-          const int64_t cb = q < ns ? p.ccol[sw.c0 + c0 + q * W + x] : -1;
+          const int64_t cb = q < ns ? ccol_off(p.ccol[sw.c0 + c0 + q * W + x]) : -1;
           if (cb >= 0 && rowok) C[cb + row] = acc[q][x];
         }
     }

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <cmath>
 #include <numeric>
+#include <unordered_set>
 
 #include "plan.hpp"
 
@@ -698,9 +699,11 @@ void build_as_schedule(sht_plan* P)
   };
   auto gen_sweeps = [&](int cap, std::vector<AsSweep>& sweeps, std::vector<int64_t>& ccol, std::vector<Row>& brows) {
     std::vector<int64_t> pcof;  // partner C offsets of the current A block
+    std::unordered_set<int64_t> seen;  // C columns of the block row touched by an earlier sweep
     int64_t t = 0;
     while (t < nt) {
       const int64_t I = aI[ta[order[t]]];
+      seen.clear();
       Row row{static_cast<int32_t>(sweeps.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
       while (t < nt && aI[ta[order[t]]] == I) {
         const int64_t a = ta[order[t]];
@@ -727,6 +730,7 @@ void build_as_schedule(sht_plan* P)
               const AsCol& cm = cols[pn.col0 + c];
               if (pcof[cm.p] >= 0) v = pcof[cm.p] + static_cast<int64_t>(cm.j) * row.rows;
             }
+            if (v >= 0 && seen.insert(v).second) v = -v - 2;  // first touch of the column
             ccol.push_back(v);
           }
           row.work += static_cast<double>(k) * 16 * sw.ns;
@@ -810,6 +814,14 @@ void build_as_schedule(sht_plan* P)
   // the blocks' own columns
   SHT_HIP(hipMemset(P->d_bt, 0, std::max<int64_t>(bt_total, 1) * bt_bytes));
   SHT_HIP(hipDeviceSynchronize());
+  {
+    std::vector<char> touched(C->indices.size(), 0);
+    for (int64_t x = 0; x < nt; ++x) touched[tc[x]] = 1;
+    P->as_all_touched = std::all_of(touched.begin(), touched.end(), [](char c) { return c != 0; });
+    P->d_as_zero = static_cast<double*>(arena.allocate(kAsZeroLen * sizeof(double)));
+    SHT_HIP(hipMemset(P->d_as_zero, 0, kAsZeroLen * sizeof(double)));
+    SHT_HIP(hipDeviceSynchronize());
+  }
   P->bt_total = bt_total;
   P->bt_nrec = static_cast<int64_t>(bt.size());
   P->as_nunits = static_cast<int64_t>(us.size());
@@ -917,7 +929,10 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   const void* Bd = P->permB ? P->d_Bp : P->B->d_data;
   void* Cd = P->permC ? P->d_Cp : P->C->d_data;
   if (P->schedule >= 1) {
-    launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
+    // beta == 0 and every C block has a task: the first sweep of every C
+    // column starts from zeros and no C = beta * C pass is needed
+    const bool zinit = bz && !az && P->as_all_touched;
+    if (!zinit) launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
     if (!az) {
       launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, P->bt_total, s);
       AsArgs g{};
@@ -933,6 +948,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.nunits = narrow ? P->as_nunits4 : (wide ? P->as_nunitsw : P->as_nunits);
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
+      g.zero = zinit ? P->d_as_zero : nullptr;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
       launch_as(cls, g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));

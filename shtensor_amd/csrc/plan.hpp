@@ -115,12 +115,20 @@ struct AsArgs {
   void* C;
   int64_t bt_total;   // elements per bf16 plane
   const AsSweep* sweeps;
-  const int64_t* ccol;  // per sweep column: element offset of C(0, column), -1 for pad columns / dropped blocks
+  // per sweep column: element offset v of C(0, column); -1 for pad columns /
+  // dropped blocks; -(v + 2) when this sweep is the first of its block row
+  // that touches the column
+  const int64_t* ccol;
   const AsUnit* units;
   int64_t nunits;
   double alpha;
   int alpha_one;
+  // non-null (beta == 0, every C block has a task): the first sweep of a
+  // column starts from zeros read here (kAsZeroLen doubles) instead of C,
+  // and no C = beta * C pass runs before the sweeps
+  const double* zero;
 };
+constexpr int kAsZeroLen = 256;
 // Kernel classes.  The MFMA classes run 32-row units (two 16-row MFMA tiles
 // per wave): Mfma64/Mfma32/Bf16x3 one wave per workgroup, the *G variants
 // kAsGroupWaves waves (units of one block row) per workgroup sharing panel
@@ -205,6 +213,8 @@ struct sht_plan {
   void* d_bt = nullptr;
   int64_t bt_nrec = 0, bt_total = 0;
   int64_t as_nunits = 0, as_nunitsw = 0, as_nunits4 = 0, as_nsweep = 0, as_npanel = 0;
+  bool as_all_touched = false;  // every C block has a task (first sweeps may initialise C)
+  double* d_as_zero = nullptr;  // kAsZeroLen zeros
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   double graph_alpha = 0.0, graph_beta = 0.0;
@@ -217,7 +227,8 @@ struct sht_plan {
                     static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_sweep),
                     static_cast<void*>(d_as_ccol), static_cast<void*>(d_as_unit),
                     static_cast<void*>(d_as_unitw), static_cast<void*>(d_as_sweep4), static_cast<void*>(d_as_ccol4),
-                    static_cast<void*>(d_as_unit4), static_cast<void*>(d_bt_rec), d_bt})
+                    static_cast<void*>(d_as_unit4), static_cast<void*>(d_bt_rec), d_bt,
+                    static_cast<void*>(d_as_zero)})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);

@@ -126,6 +126,44 @@ def test_alpha_beta_kept_pattern(ctx, prec):
         assert normwise(out, R.data) <= 1e-12
 
 
+@pytest.mark.parametrize("extra", [False, True])
+@pytest.mark.parametrize("prec", ["fp64", "fp64_as1", "fp64_as2", "fp64_as3", "fp64_exact_as1", "fp32_as2",
+                                  "bf16x3_as2"])
+def test_beta_zero_does_not_read_c(ctx, prec, extra):
+    """beta == 0: C's previous contents (NaN here) are never read.  With
+    extra C blocks that no task reaches, those blocks become zero; without,
+    the A-stationary schedules start every column's first sweep from zeros
+    instead of a C = 0 pass."""
+    print("beta0", prec, extra, flush=True)
+    rng = np.random.default_rng(11)
+    expr = "ijkl,klm->ijm"
+    A, B, Cp = make_case(rng, "ijkl", "klm", "ijm", p=0.6, bmin=2, bmax=9)
+    _, _, cids = orc.tasks(expr, A, B, Cp)
+    ids = cids
+    if extra:
+        total = int(np.prod([len(b) for b in Cp.block_sizes]))
+        free = np.setdiff1d(np.arange(total), cids)
+        assert len(free) > 0
+        ids = np.union1d(cids, free[:: max(1, len(free) // 5)])
+    Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(ids)
+    c0 = np.full(len(Ck.data), np.nan)
+    Ck.data[:] = c0
+    _, _, _, R = oracle_result(expr, A, B, Ck, alpha=1.0, beta=0.0, c0=c0)
+    assert not np.isnan(R.data).any()
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = to_device(ctx, Ck)
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
+    plan.execute(1.0, 0.0)
+    sht.call("sht_device_synchronize")
+    plan.close()
+    out = dC.download()
+    assert not np.isnan(out).any()
+    if prec in EXACT:
+        assert np.array_equal(out, R.data)
+    else:
+        assert normwise(out, R.data) <= TOL[prec]
+
+
 def test_alpha_zero_scales_only(ctx):
     rng = np.random.default_rng(8)
     A, B, Cp = make_case(rng, "ik", "kj", "ij", p=0.8)
