@@ -698,8 +698,16 @@ __device__ inline void run_sweep(const AsArgs& p, const SweepU& sw, const SweepU
 // Waves are independent (no workgroup barrier): an empty unit just returns.
 // The records of the next two sweeps and the next sweep's column offsets are
 // loaded one sweep ahead, so a sweep starts without a dependent load chain.
+// The split-BF16 instances (second translation unit) hold 64 AGPR
+// accumulators next to about 190 VGPRs: they are held to two waves per
+// SIMD (256 registers in all), which their speed depends on.
+#ifdef SHT_AS_BF3_TU
+#define SHT_ASP_OCC __attribute__((amdgpu_waves_per_eu(2)))
+#else
+#define SHT_ASP_OCC
+#endif
 template <int R, int NSMAX, int F, int WPG, bool A1>
-__global__ __launch_bounds__(64 * WPG) void k_asp(AsArgs p)
+__global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
 {
   static_assert(16 * NSMAX <= 128, "two column offsets per lane");
   const int wave = WPG == 1 ? 0 : uni(static_cast<int>(threadIdx.x >> 6));
