@@ -225,7 +225,11 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
   nx.a = static_cast<const double*>(p.A) + nxs.a_off;
   nx.K = nxs.k;
   nx.M = nxs.m;
+  // steps past the contracted length (the loop runs whole 16-k iterations)
+  // issue no MFMAs: every product they hold is zero
+  const int kq = (K + 3) >> 2;
   auto mma = [&](int st, const double(&araw)[R], const TB(&b)[NS]) {
+    if (st >= kq) return;
     const bool kin = 4 * st + lk < K;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -348,6 +352,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // rows are masked (k >= K) and re-read its last row
   const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
   const int nstepsU = (nsteps + AD - 1) / AD * AD;
+  const int kq = (K + 3) >> 2;  // steps holding contracted indices < K
   // Every global load below is unconditional: a load under a branch makes
   // the compiler drain all outstanding loads where the paths merge, which
   // would discard the A and panel prefetch.  Ghost waves load their share of
@@ -443,7 +448,9 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
       arun += a4m;
     }
     __builtin_amdgcn_sched_barrier(0);
-    if (!ghost) {
+    // (steps past the contracted length issue no MFMAs: their products are
+    // all zero; the loads, LDS writes and barriers above and below still run)
+    if (!ghost && st + j < kq) {
 #pragma unroll
       for (int q = 0; q < NS; ++q) b[q] = stage[j][q][lane];
       mma(st + j, ar[j]);
