@@ -1012,11 +1012,6 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
       sht::build_tasks(P.get());
       sht::build_schedule(P.get());
       sht::setup_staging(P.get());
-      const char* e = std::getenv("SHT_SCHED");
-      if (e && std::string(e) == "as" && sht::as_supported(P.get())) {
-        sht::build_as_schedule(P.get());
-        P->schedule = 1;
-      }
     } catch (...) {
       P->release();
       throw;
