@@ -16,6 +16,8 @@
 //  * k_bf16x3<CT,RT> — three-term split-BF16 (x = hi + mid + lo), six
 //    products of order <= 2 with v_mfma_f32_16x16x32_bf16, FP32 per task,
 //    FP64 across.
+//  * k_mfma64w — FP64 MFMA, four waves per 64 x 64 tile sharing every
+//    A and B chunk through LDS (C blocks more than 32 columns wide).
 //  RT = 4 (64-row tiles) in general; RT = 2 (32-row tiles) for a block's
 //  last row tile when at most 32 rows remain, so blocks of <= 32 rows (and
 //  the remainders of larger ones) issue no MFMAs on padding rows.
@@ -27,6 +29,7 @@ namespace sht {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef short v8s __attribute__((ext_vector_type(8)));
+typedef double v2d8 __attribute__((ext_vector_type(2), aligned(8)));  // 16 bytes, 8-byte aligned
 
 namespace {
 
@@ -194,6 +197,132 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
       }
       wave_lds_sync();
     }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA, one workgroup of four waves per 64 x 64 C tile (C blocks more
+// than 32 columns wide).  The workgroup walks the block's tasks in 16-k
+// chunks: each chunk of A (64 rows x 16) and B (16 x 64 columns) is loaded
+// once, by all 256 lanes in 32-byte pieces, staged in LDS (two stages, one
+// barrier per chunk) and read by the waves as MFMA fragments; wave w owns
+// rows 32(w & 1) .. +31 and columns 32(w >> 1) .. +31 of the tile (2 x 2
+// transposed MFMA tiles, as k_mfma64).  The next chunk's global loads are
+// issued before the current chunk's MFMAs.  Loads are unconditional: rows
+// past M, columns past N and contracted indices past K read clamped
+// in-bounds addresses (tensor buffers carry slack); k >= K is zeroed when
+// the chunk is written to LDS, rows and columns past the block are never
+// stored.  Chunks run task after task in task order and k ascending, so
+// every C element sees the same fma chain as in the other FP64 kernels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
+{
+  const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // (uniform over the workgroup)
+  const CBlkRec cb = p.cblk[tile.cb];
+  const int M = cb.m, N = cb.n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  double* Cb = static_cast<double*>(p.C) + cb.c_off;
+  __shared__ double sA[2][16][64];  // [stage][k][row of the tile]
+  __shared__ double sB[2][16][64];  // [stage][k][column of the tile]
+  v4d acc[2][2];
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + wr + 16 * rt + li;
+        const int j = tile.n0 + wc + 16 * ct + 4 * v + lk;
+        acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
+      }
+  if (!p.alpha_zero && cb.t1 > cb.t0) {
+    // loader lanes: A column kA (0..15) rows rA..rA+3; B column jB (0..63)
+    // contracted rows kB..kB+3
+    const int kA = tid >> 4, rA = 4 * (tid & 15);
+    const int jB = tid >> 2, kB = 4 * (tid & 3);
+    const int ia = min(tile.m0 + rA, M - 1);
+    const int jb = min(tile.n0 + jB, N - 1);
+    int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
+    TaskRec tr = p.tasks[tl];
+    double ra[4], rb[4];
+    bool ka;     // this lane's A column is inside the task
+    int kbb, Kb;  // contracted index of rb[0] and the task's K
+    const double alpha = p.alpha;
+    const bool a1 = p.alpha_one != 0;
+#define SHT_W_LOAD()                                                                       \
+  {                                                                                        \
+    const int K = tr.k;                                                                    \
+    const int kk = kl + kA;                                                                \
+    ka = kk < K;                                                                           \
+    const double* pa = A + tr.a_off + static_cast<int64_t>(min(kk, K - 1)) * M + ia;       \
+    const v2d8 a01 = *reinterpret_cast<const v2d8*>(pa);                                   \
+    const v2d8 a23 = *reinterpret_cast<const v2d8*>(pa + 2);                               \
+    kbb = kl + kB;                                                                         \
+    Kb = K;                                                                                \
+    const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(kbb, K - 1);      \
+    const v2d8 b01 = *reinterpret_cast<const v2d8*>(pb);                                   \
+    const v2d8 b23 = *reinterpret_cast<const v2d8*>(pb + 2);                               \
+    ra[0] = a01.x; ra[1] = a01.y; ra[2] = a23.x; ra[3] = a23.y;                            \
+    rb[0] = b01.x; rb[1] = b01.y; rb[2] = b23.x; rb[3] = b23.y;                            \
+    if (tl < cb.t1) {                                                                      \
+      kl += 16;                                                                            \
+      if (kl >= K) {                                                                       \
+        kl = 0;                                                                            \
+        ++tl;                                                                              \
+        if (tl < cb.t1) tr = p.tasks[tl];                                                  \
+      }                                                                                    \
+    }                                                                                      \
+  }
+#define SHT_W_STORE(stg)                                                                   \
+  {                                                                                        \
+    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                        \
+      const double x = ka ? ra[e] : 0.0;                                                   \
+      sA[stg][kA][rA + e] = a1 ? x : x * alpha;                                            \
+      sB[stg][kB + e][jB] = kbb + e < Kb ? rb[e] : 0.0;                                    \
+    }                                                                                      \
+  }
+    SHT_W_LOAD()
+    SHT_W_STORE(0)
+    __syncthreads();
+    int stg = 0;
+    for (;;) {
+      const bool more = tl < cb.t1;  // a next chunk exists (uniform)
+      SHT_W_LOAD()                   // (past the end: a clamped re-read, unused)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        double a[2], b[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) a[rt] = sA[stg][4 * s + lk][wr + 16 * rt + li];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) b[ct] = sB[stg][4 * s + lk][wc + 16 * ct + li];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct)
+            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
+      }
+      if (!more) break;
+      SHT_W_STORE(stg ^ 1)
+      __syncthreads();
+      stg ^= 1;
+    }
+#undef SHT_W_LOAD
+#undef SHT_W_STORE
+  }
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = tile.m0 + wr + 16 * rt + li;
+        const int j = tile.n0 + wc + 16 * ct + 4 * v + lk;
+        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
+      }
 }
 
 // ---------------------------------------------------------------------------
@@ -444,7 +573,7 @@ int kclass_rows(KClass k)
     case KClass::F32x32h:
     case KClass::Bf16x16h:
     case KClass::Bf16x32h: return 32;
-    default: return 64;
+    default: return 64;  // (Mfma64W: 64)
   }
 }
 
@@ -460,6 +589,7 @@ int kclass_cols(KClass k)
     case KClass::Mfma16h:
     case KClass::F32x16h:
     case KClass::Bf16x16h: return 16;
+    case KClass::Mfma64W: return 64;
     default: return 32;
   }
 }
@@ -484,6 +614,7 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::F32x32h: hipLaunchKernelGGL((k_mfma32<2, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x16h: hipLaunchKernelGGL((k_bf16x3<1, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x32h: hipLaunchKernelGGL((k_bf16x3<2, 2>), g, b, 0, s, a); break;
+    case KClass::Mfma64W: hipLaunchKernelGGL(k_mfma64w, g, dim3(4 * kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
   SHT_HIP(hipGetLastError());

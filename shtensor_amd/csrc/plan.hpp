@@ -54,6 +54,7 @@ enum class KClass : int {
   Mfma16h, Mfma32h,                  // the three MFMA forms above on 32-row
   F32x16h, F32x32h,                  // tiles (a block's last row tile when
   Bf16x16h, Bf16x32h,                // at most 32 rows remain)
+  Mfma64W,                           // FP64 MFMA, 64 x 64 tiles, four waves per tile
   Count
 };
 

@@ -278,6 +278,34 @@ def test_autotune_keeps_results(ctx):
     plan.close()
 
 
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.5, -1.25)])
+def test_wide_blocks_c_stationary(ctx, alpha, beta):
+    """C blocks of 33..130 columns run on the four-wave 64 x 64 tiles (A and
+    B chunks shared through LDS), with partial tiles in both directions and
+    contracted lengths that are not multiples of 16: bit for bit equal to the
+    oracle's fma chain."""
+    rng = np.random.default_rng(78)
+    expr = "ik,kj->ij"
+    bs_i = [16, 50, 64, 100, 7]
+    bs_k = [3, 16, 37, 64]
+    bs_j = [33, 48, 64, 80, 130, 20]
+    A = random_bst(rng, [bs_i, bs_k], 0.8)
+    B = random_bst(rng, [bs_k, bs_j], 0.8)
+    Cp = orc.BlockSparse([bs_i, bs_j])
+    _, _, cids = orc.tasks(expr, A, B, Cp)
+    Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+    c0 = rng.uniform(-1, 1, len(Ck.data))
+    _, _, _, R = oracle_result(expr, A, B, Ck, alpha=alpha, beta=beta, c0=c0)
+    Ck.data[:] = c0
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = to_device(ctx, Ck)
+    plan = make_plan(ctx, expr, dA, dB, dC, "fp64")
+    plan.execute(alpha, beta)
+    sht.call("sht_device_synchronize")
+    plan.close()
+    assert np.array_equal(dC.download(), R.data)
+
+
 @pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (1, 2, 3))])
 def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
     """Block rows of 1..7 32-row chunks (so the grouped schedule pads its
