@@ -16,11 +16,14 @@
 //  * k_bf16x3<CT,RT> — three-term split-BF16 (x = hi + mid + lo), six
 //    products of order <= 2 with v_mfma_f32_16x16x32_bf16, FP32 per task,
 //    FP64 across.
-//  * k_mfma64w — FP64 MFMA, four waves per 64 x 64 tile sharing every
-//    A and B chunk through LDS (C blocks more than 32 columns wide).
+//  * k_mfmaw<F> — FP64 (F = 0) or FP32-per-task (F = 1) MFMA, four waves
+//    per 64 x 64 tile sharing every A and B chunk through LDS (C blocks
+//    more than 32 columns wide).
 //  RT = 4 (64-row tiles) in general; RT = 2 (32-row tiles) for a block's
 //  last row tile when at most 32 rows remain, so blocks of <= 32 rows (and
 //  the remainders of larger ones) issue no MFMAs on padding rows.
+
+#include <type_traits>
 
 #include "plan.hpp"
 
@@ -200,22 +203,26 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 }
 
 // ---------------------------------------------------------------------------
-// FP64 MFMA, one workgroup of four waves per 64 x 64 C tile (C blocks more
-// than 32 columns wide).  The workgroup walks the block's tasks in 16-k
-// chunks: each chunk of A (64 rows x 16) and B (16 x 64 columns) is loaded
-// once, by all 256 lanes in 32-byte pieces, staged in LDS (two stages, one
-// barrier per chunk) and read by the waves as MFMA fragments; wave w owns
-// rows 32(w & 1) .. +31 and columns 32(w >> 1) .. +31 of the tile (2 x 2
-// transposed MFMA tiles, as k_mfma64).  The next chunk's global loads are
-// issued before the current chunk's MFMAs.  Loads are unconditional: rows
-// past M, columns past N and contracted indices past K read clamped
-// in-bounds addresses (tensor buffers carry slack); k >= K is zeroed when
-// the chunk is written to LDS, rows and columns past the block are never
-// stored.  Chunks run task after task in task order and k ascending, so
-// every C element sees the same fma chain as in the other FP64 kernels.
+// MFMA on 64 x 64 C tiles, one workgroup of four waves per tile (C blocks
+// more than 32 columns wide); F = 0: FP64 throughout, F = 1: FP32 per task
+// (products and sums of one block product), FP64 across tasks.  The
+// workgroup walks the block's tasks in 16-k chunks: each chunk of A (64
+// rows x 16) and B (16 x 64 columns) is loaded once, by all 256 lanes in
+// 32-byte pieces, staged in LDS (two stages, one barrier per chunk) and read
+// by the waves as MFMA fragments; wave w owns rows 32(w & 1) .. +31 and
+// columns 32(w >> 1) .. +31 of the tile (2 x 2 transposed MFMA tiles, as
+// k_mfma64 / k_mfma32).  The next chunk's global loads are issued before
+// the current chunk's MFMAs.  Loads are unconditional: rows past M, columns
+// past N and contracted indices past K read clamped in-bounds addresses
+// (tensor buffers carry slack); k >= K is zeroed when the chunk is written
+// to LDS, rows and columns past the block are never stored.  Chunks run task
+// after task in task order and k ascending, so every C element sees the
+// same chain as in the one-wave kernels of the same precision.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
+template <int F>
+__global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 {
+  typedef typename std::conditional<F == 0, double, float>::type TL;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
   const CBlkRec cb = p.cblk[tile.cb];
@@ -226,8 +233,10 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ double sA[2][16][64];  // [stage][k][row of the tile]
-  __shared__ double sB[2][16][64];  // [stage][k][column of the tile]
+  __shared__ TL sA[2][16][64];  // [stage][k][row of the tile]
+  __shared__ TL sB[2][16][64];  // [stage][k][column of the tile]
+  // column of accumulator element v (the f64 and f32 MFMA results differ)
+  auto colof = [&](int ct, int v) { return tile.n0 + wc + 16 * ct + (F == 0 ? 4 * v + lk : 4 * lk + v); };
   v4d acc[2][2];
 #pragma unroll
   for (int rt = 0; rt < 2; ++rt)
@@ -236,7 +245,7 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
-        const int j = tile.n0 + wc + 16 * ct + 4 * v + lk;
+        const int j = colof(ct, v);
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero && cb.t1 > cb.t0) {
@@ -249,10 +258,16 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
     TaskRec tr = p.tasks[tl];
     double ra[4], rb[4];
-    bool ka;     // this lane's A column is inside the task
-    int kbb, Kb;  // contracted index of rb[0] and the task's K
+    bool ka;       // this lane's A column is inside the task
+    int kbb, Kb;   // contracted index of rb[0] and the task's K
+    bool lastl;    // the loaded chunk is the last of its task (uniform)
     const double alpha = p.alpha;
     const bool a1 = p.alpha_one != 0;
+    v4f part[2][2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
 #define SHT_W_LOAD()                                                                       \
   {                                                                                        \
     const int K = tr.k;                                                                    \
@@ -268,6 +283,7 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
     const v2d8 b23 = *reinterpret_cast<const v2d8*>(pb + 2);                               \
     ra[0] = a01.x; ra[1] = a01.y; ra[2] = a23.x; ra[3] = a23.y;                            \
     rb[0] = b01.x; rb[1] = b01.y; rb[2] = b23.x; rb[3] = b23.y;                            \
+    lastl = kl + 16 >= K;                                                                  \
     if (tl < cb.t1) {                                                                      \
       kl += 16;                                                                            \
       if (kl >= K) {                                                                       \
@@ -281,12 +297,19 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
   {                                                                                        \
     _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                        \
       const double x = ka ? ra[e] : 0.0;                                                   \
-      sA[stg][kA][rA + e] = a1 ? x : x * alpha;                                            \
-      sB[stg][kB + e][jB] = kbb + e < Kb ? rb[e] : 0.0;                                    \
+      const double y = kbb + e < Kb ? rb[e] : 0.0;                                         \
+      if constexpr (F == 0) {                                                              \
+        sA[stg][kA][rA + e] = a1 ? x : x * alpha;                                          \
+        sB[stg][kB + e][jB] = y;                                                           \
+      } else {                                                                             \
+        sA[stg][kA][rA + e] = static_cast<float>(x);                                       \
+        sB[stg][kB + e][jB] = static_cast<float>(y);                                       \
+      }                                                                                    \
     }                                                                                      \
   }
     SHT_W_LOAD()
     SHT_W_STORE(0)
+    bool lastc = lastl;  // the chunk in the current stage ends its task
     __syncthreads();
     int stg = 0;
     for (;;) {
@@ -294,7 +317,7 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
       SHT_W_LOAD()                   // (past the end: a clamped re-read, unused)
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        double a[2], b[2];
+        TL a[2], b[2];
 #pragma unroll
         for (int rt = 0; rt < 2; ++rt) a[rt] = sA[stg][4 * s + lk][wr + 16 * rt + li];
 #pragma unroll
@@ -302,11 +325,28 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
 #pragma unroll
         for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct)
-            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
+          for (int ct = 0; ct < 2; ++ct) {
+            if constexpr (F == 0)
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
+            else
+              part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[ct], a[rt], part[rt][ct], 0, 0, 0);
+          }
+      }
+      if constexpr (F == 1) {
+        if (lastc) {  // the task's block product is complete: add it in FP64
+#pragma unroll
+          for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) acc[rt][ct][v] += alpha * static_cast<double>(part[rt][ct][v]);
+              part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        }
       }
       if (!more) break;
       SHT_W_STORE(stg ^ 1)
+      lastc = lastl;
       __syncthreads();
       stg ^= 1;
     }
@@ -320,7 +360,7 @@ __global__ __launch_bounds__(256) void k_mfma64w(ExecArgs p)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
-        const int j = tile.n0 + wc + 16 * ct + 4 * v + lk;
+        const int j = colof(ct, v);
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
 }
@@ -589,7 +629,8 @@ int kclass_cols(KClass k)
     case KClass::Mfma16h:
     case KClass::F32x16h:
     case KClass::Bf16x16h: return 16;
-    case KClass::Mfma64W: return 64;
+    case KClass::Mfma64W:
+    case KClass::F32W: return 64;
     default: return 32;
   }
 }
@@ -614,7 +655,8 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::F32x32h: hipLaunchKernelGGL((k_mfma32<2, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x16h: hipLaunchKernelGGL((k_bf16x3<1, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x32h: hipLaunchKernelGGL((k_bf16x3<2, 2>), g, b, 0, s, a); break;
-    case KClass::Mfma64W: hipLaunchKernelGGL(k_mfma64w, g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64W: hipLaunchKernelGGL(k_mfmaw<0>, g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32W: hipLaunchKernelGGL(k_mfmaw<1>, g, dim3(4 * kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
   SHT_HIP(hipGetLastError());

@@ -504,7 +504,7 @@ void build_schedule(sht_plan* P)
     const int M = cb[c].m, N = cb[c].n;
     KClass k;
     if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
-    else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : KClass::F32x32;
+    else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : (N <= 32 ? KClass::F32x32 : KClass::F32W);
     else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : KClass::Bf16x32;
     else k = N <= 16 ? KClass::Mfma16 : (N <= 32 ? KClass::Mfma32 : KClass::Mfma64W);
     const int rows = kclass_rows(k), cols = kclass_cols(k);
@@ -1070,7 +1070,7 @@ const char* sht_kclass_name(int k)
   static const char* names[] = {"exact_n8",      "exact_n16",     "exact_n32",     "mfma64_n16",
                                 "mfma64_n32",    "mfma32_n16",    "mfma32_n32",    "bf16x3_n16",
                                 "bf16x3_n32",    "mfma64_n16_r32", "mfma64_n32_r32", "mfma32_n16_r32",
-                                "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32", "mfma64_n64w"};
+                                "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32", "mfma64_n64w", "mfma32_n64w"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
   return names[k];
 }

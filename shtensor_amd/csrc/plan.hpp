@@ -55,6 +55,7 @@ enum class KClass : int {
   F32x16h, F32x32h,                  // tiles (a block's last row tile when
   Bf16x16h, Bf16x32h,                // at most 32 rows remain)
   Mfma64W,                           // FP64 MFMA, 64 x 64 tiles, four waves per tile
+  F32W,                              // FP32 MFMA (FP64 cross-task), 64 x 64 tiles, four waves
   Count
 };
 

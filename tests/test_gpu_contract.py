@@ -278,12 +278,13 @@ def test_autotune_keeps_results(ctx):
     plan.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.5, -1.25)])
-def test_wide_blocks_c_stationary(ctx, alpha, beta):
+def test_wide_blocks_c_stationary(ctx, alpha, beta, prec):
     """C blocks of 33..130 columns run on the four-wave 64 x 64 tiles (A and
     B chunks shared through LDS), with partial tiles in both directions and
-    contracted lengths that are not multiples of 16: bit for bit equal to the
-    oracle's fma chain."""
+    contracted lengths that are not multiples of 16: FP64 bit for bit equal
+    to the oracle's fma chain, FP32 within 1e-6."""
     rng = np.random.default_rng(78)
     expr = "ik,kj->ij"
     bs_i = [16, 50, 64, 100, 7]
@@ -299,11 +300,14 @@ def test_wide_blocks_c_stationary(ctx, alpha, beta):
     Ck.data[:] = c0
     dA, dB = to_device(ctx, A), to_device(ctx, B)
     dC = to_device(ctx, Ck)
-    plan = make_plan(ctx, expr, dA, dB, dC, "fp64")
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
     plan.execute(alpha, beta)
     sht.call("sht_device_synchronize")
     plan.close()
-    assert np.array_equal(dC.download(), R.data)
+    if prec in EXACT:
+        assert np.array_equal(dC.download(), R.data)
+    else:
+        assert normwise(dC.download(), R.data) <= TOL[prec]
 
 
 @pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (1, 2, 3))])
