@@ -180,11 +180,13 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
   }
   // Write-back four columns at a time through LDS: lane (li, lk) holds rows
   // 16rt + li of column 4v + lk of the group; lane l then stores row
-  // m0 + l % R of column l / R of each group of 64 / R columns, so every
-  // store instruction writes whole R-row column pieces (64 x 8 bytes).
+  // m0 + l % R of column l / R of each group of 64 / R columns (R <= 64), or
+  // rows m0 + 64h + l of each column (R > 64), so every store instruction
+  // writes whole column pieces of 64 x 8 bytes.
   constexpr int R = 16 * RT;
+  constexpr int CPS = R >= 64 ? 1 : 64 / R;  // columns per store instruction
+  constexpr int RPS = R >= 64 ? R / 64 : 1;  // store instructions per column
   __shared__ double cst[4 * R];
-  const int i = tile.m0 + lane % R;
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
@@ -193,11 +195,15 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
       for (int rt = 0; rt < RT; ++rt) cst[lk * R + 16 * rt + li] = acc[rt][ct][v];
       wave_lds_sync();
 #pragma unroll
-      for (int c = 0; c < 4; c += 64 / R) {
-        const int cc = c + lane / R;
-        const int j = tile.n0 + 16 * ct + 4 * v + cc;
-        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[cc * R + lane % R];
-      }
+      for (int c = 0; c < 4; c += CPS)
+#pragma unroll
+        for (int h = 0; h < RPS; ++h) {
+          const int cc = R >= 64 ? c : c + lane / R;
+          const int r = R >= 64 ? 64 * h + lane : lane % R;
+          const int i = tile.m0 + r;
+          const int j = tile.n0 + 16 * ct + 4 * v + cc;
+          if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[cc * R + r];
+        }
       wave_lds_sync();
     }
 }
@@ -613,6 +619,8 @@ int kclass_rows(KClass k)
     case KClass::F32x32h:
     case KClass::Bf16x16h:
     case KClass::Bf16x32h: return 32;
+    case KClass::Mfma16q:
+    case KClass::Mfma32q: return 128;
     default: return 64;  // (Mfma64W: 64)
   }
 }
@@ -628,7 +636,8 @@ int kclass_cols(KClass k)
     case KClass::Bf16x16:
     case KClass::Mfma16h:
     case KClass::F32x16h:
-    case KClass::Bf16x16h: return 16;
+    case KClass::Bf16x16h:
+    case KClass::Mfma16q: return 16;
     case KClass::Mfma64W:
     case KClass::F32W: return 64;
     default: return 32;
@@ -655,6 +664,8 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::F32x32h: hipLaunchKernelGGL((k_mfma32<2, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x16h: hipLaunchKernelGGL((k_bf16x3<1, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x32h: hipLaunchKernelGGL((k_bf16x3<2, 2>), g, b, 0, s, a); break;
+    case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
+    case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
     case KClass::Mfma64W: hipLaunchKernelGGL(k_mfmaw<0>, g, dim3(4 * kWave), 0, s, a); break;
     case KClass::F32W: hipLaunchKernelGGL(k_mfmaw<1>, g, dim3(4 * kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");

@@ -441,6 +441,16 @@ KClass half_rows(KClass k)
   }
 }
 
+// The 128-row form of a 64-row FP64 MFMA tile class (others: unchanged).
+KClass quad_rows(KClass k)
+{
+  switch (k) {
+    case KClass::Mfma16: return KClass::Mfma16q;
+    case KClass::Mfma32: return KClass::Mfma32q;
+    default: return k;
+  }
+}
+
 void build_schedule(sht_plan* P)
 {
   Arena& arena = P->ctx->arena;
@@ -500,6 +510,12 @@ void build_schedule(sht_plan* P)
   };
   std::vector<std::vector<T>> per(static_cast<int>(KClass::Count));
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
+  // 128-row FP64 tiles only for large launches (more than 64 64-row tiles
+  // per SIMD): there a wave's fixed cost (the dependent record loads before
+  // its first MFMA) dominates, while small launches need every wave
+  int64_t n64 = 0;
+  for (int64_t c = 0; c < ncb; ++c) n64 += (cb[c].m + 63) / 64;
+  const bool quad = P->prec == SHT_PREC_FP64 && n64 > 64 * 1024;
   for (int64_t c = 0; c < ncb; ++c) {
     const int M = cb[c].m, N = cb[c].n;
     KClass k;
@@ -507,16 +523,21 @@ void build_schedule(sht_plan* P)
     else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : (N <= 32 ? KClass::F32x32 : KClass::F32W);
     else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : KClass::Bf16x32;
     else k = N <= 16 ? KClass::Mfma16 : (N <= 32 ? KClass::Mfma32 : KClass::Mfma64W);
-    const int rows = kclass_rows(k), cols = kclass_cols(k);
+    const int cols = kclass_cols(k);
     const int64_t rk = row_key(P->C->indices[c]);
-    for (int m0 = 0; m0 < M; m0 += rows) {
-      // the last row tile of an MFMA class takes the 32-row form when it can
-      const KClass kt = (!exact && M - m0 <= 32) ? half_rows(k) : k;
+    for (int m0 = 0; m0 < M;) {
+      // row tiles: 128 rows while more than 96 remain (FP64 classes, large
+      // launches), then 64; the last row tile of an MFMA class takes the
+      // 32-row form when it can
+      KClass kt = k;
+      if (!exact && M - m0 <= 32) kt = half_rows(k);
+      else if (quad && M - m0 > 96) kt = quad_rows(k);
       for (int n0 = 0; n0 < N; n0 += cols) {
         const double cost =
             static_cast<double>(std::min(kclass_rows(kt), M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
         per[static_cast<int>(kt)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost, rk});
       }
+      m0 += kclass_rows(kt);
     }
   }
   constexpr int kXcd = 8;
@@ -1070,7 +1091,8 @@ const char* sht_kclass_name(int k)
   static const char* names[] = {"exact_n8",      "exact_n16",     "exact_n32",     "mfma64_n16",
                                 "mfma64_n32",    "mfma32_n16",    "mfma32_n32",    "bf16x3_n16",
                                 "bf16x3_n32",    "mfma64_n16_r32", "mfma64_n32_r32", "mfma32_n16_r32",
-                                "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32", "mfma64_n64w", "mfma32_n64w"};
+                                "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32", "mfma64_n64w", "mfma32_n64w", "mfma64_n16_r128",
+                                "mfma64_n32_r128"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
   return names[k];
 }

@@ -310,6 +310,31 @@ def test_wide_blocks_c_stationary(ctx, alpha, beta, prec):
         assert normwise(dC.download(), R.data) <= TOL[prec]
 
 
+def test_large_launch_128_row_tiles(ctx):
+    """A launch of more than 64 * 1024 64-row tiles runs FP64 C blocks of
+    more than 96 rows on 128-row tiles (then 64- and 32-row tiles for the
+    rest): 38 400 C blocks of 20..230 rows and 4 or 20 columns, bit for bit
+    equal to the oracle."""
+    rng = np.random.default_rng(79)
+    expr = "ik,kj->ij"
+    bs_i = [100, 230, 40, 20] * 32
+    bs_k = [8, 16, 5, 3]
+    bs_j = [4, 20] * 150
+    A = random_bst(rng, [bs_i, bs_k], 1.0)
+    B = random_bst(rng, [bs_k, bs_j], 1.0)
+    Cp = orc.BlockSparse([bs_i, bs_j])
+    _, _, _, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    plan = make_plan(ctx, expr, dA, dB, dC, "fp64")
+    launches = [k for k, _ in plan.execute_timed(1.0, 0.0)]
+    assert "mfma64_n16_r128" in launches and "mfma64_n32_r128" in launches, launches
+    sht.call("sht_device_synchronize")
+    plan.close()
+    assert np.array_equal(dC.download(), R.data)
+    dC.close()
+
+
 @pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (1, 2, 3))])
 def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
     """Block rows of 1..7 32-row chunks (so the grouped schedule pads its
