@@ -6,7 +6,7 @@ import pytest
 
 import shtensor_amd as sht
 from oracle import oracle as orc
-from gpu_util import make_case, to_device
+from gpu_util import make_case, random_bst, to_device
 
 pytestmark = pytest.mark.gpu
 
@@ -78,6 +78,39 @@ def test_mix_matches_oracle(ctx):
         else:
             assert np.array_equal(got, R.data)
     srv.close()
+
+
+def test_wide_blocks_merged(ctx):
+    """Requests with C blocks wider than 32 columns (four-wave 64 x 64
+    tiles) and narrow ones share launches; each result equals the same
+    contraction run alone."""
+    rng = np.random.default_rng(13)
+    reqs = []
+    for r in range(4):
+        bs_i = [int(x) for x in rng.integers(20, 90, 3)]
+        bs_k = [int(x) for x in rng.integers(3, 40, 3)]
+        bs_j = [int(x) for x in rng.integers(33, 80, 3)] if r % 2 == 0 else [int(x) for x in rng.integers(4, 30, 3)]
+        A = random_bst(rng, [bs_i, bs_k], 0.7)
+        B = random_bst(rng, [bs_k, bs_j], 0.7)
+        reqs.append((A, B, orc.BlockSparse([bs_i, bs_j]), [sht.PREC_FP64, sht.PREC_FP32][(r // 2) % 2]))
+    alone, dev = [], []
+    for A, B, Cp, prec in reqs:
+        dA, dB = to_device(ctx, A), to_device(ctx, B)
+        dC = sht.Tensor(ctx, Cp.block_sizes)
+        sht.contract(ctx, "ik,kj->ij", 1.0, dA, dB, 0.0, dC, prec)
+        alone.append(dC.download())
+        dev.append((dA, dB, Cp, prec))
+    srv = sht.Server(ctx, 2 << 30, max_batch=8)
+    outs, ids = [], []
+    for dA, dB, Cp, prec in dev:
+        dC = sht.Tensor(ctx, Cp.block_sizes)
+        outs.append(dC)
+        ids.append(srv.submit("ik,kj->ij", 1.0, dA, dB, 0.0, dC, prec))
+    for i in ids:
+        srv.wait(i)
+    srv.close()
+    for data, dC in zip(alone, outs):
+        assert np.array_equal(dC.download(), data)
 
 
 def test_admission_rejects_oversized(ctx):
