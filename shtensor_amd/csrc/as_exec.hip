@@ -763,6 +763,194 @@ __global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
   }
 }
 
+#ifdef SHT_AS_BF3_TU
+// ---------------------------------------------------------------------------
+// Split-BF16 for four waves over four 32-row chunks of one block row (ghost
+// units pad the workgroup).  Each 32-k step's panel fragments -- three bf16
+// planes of 16 columns per slot, 16 bytes per lane and plane -- are loaded
+// once per workgroup (wave w: slots w and w + 4) one step ahead and staged
+// through two LDS stages, one barrier per step.  The raw A values of the next
+// step are loaded (unconditionally; ghosts read one fixed line) while the
+// current step's products run, and are masked (k >= K) and split when
+// consumed.  Products, their order and the FP32-per-sweep / FP64-C update are
+// those of sweep_bf3, so the results are the same.
+// ---------------------------------------------------------------------------
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+template <int NS>
+__device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, int li, int lk, int wave, bool ghost,
+                                     const int64_t* cbs, v4u (*stage)[3][kAsSweepSlots][64])
+{
+  constexpr int R = 2;
+  constexpr int NSW = (NS + 3) / 4;
+  const int lane = li + 16 * lk;
+  const double* A = static_cast<const double*>(p.A) + sw.a_off;
+  const int kp = sw.pitch;
+  const unsigned short* Bl =
+      static_cast<const unsigned short*>(p.B) + sw.b_off + static_cast<int64_t>(li) * kp + 8 * lk;
+  const int64_t plane = p.bt_total;
+  const int64_t qstep = 16 * static_cast<int64_t>(kp);
+  double* C = static_cast<double*>(p.C);
+  const int K = sw.k, M = sw.m;
+  const int rl = r0 + li;
+  const int nsteps = (K + kBfK - 1) / kBfK;
+  const int nst2 = (nsteps + 1) & ~1;  // the loop runs whole pairs of steps
+  v4f acc[R][NS];
+#pragma unroll
+  for (int t = 0; t < R; ++t)
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
+  // this wave's panel share (a share past the sweep's slots re-reads slot
+  // NS - 1 into an unused stage slot)
+  int64_t soff[NSW];
+#pragma unroll
+  for (int x = 0; x < NSW; ++x) soff[x] = min(wave + 4 * x, NS - 1) * qstep;
+  v4u g[NSW][3];
+  auto loadP = [&](int st) {
+    const unsigned short* b = Bl + kBfK * min(st, nsteps - 1);
+#pragma unroll
+    for (int x = 0; x < NSW; ++x)
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) g[x][pl] = *reinterpret_cast<const v4u*>(b + pl * plane + soff[x]);
+  };
+  auto writeP = [&](int stg) {
+#pragma unroll
+    for (int x = 0; x < NSW; ++x)
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) stage[stg][pl][wave + 4 * x][lane] = g[x][pl];
+  };
+  // raw A of step st: rows rl, rl + 16, contracted indices 32st + 8lk + e
+  auto loadA = [&](int st, double (&x)[R][8]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int kk = kBfK * min(st, nsteps - 1) + 8 * lk + e;
+      const uint32_t off = ghost ? static_cast<uint32_t>(rl) : static_cast<uint32_t>(min(kk, K - 1) * M + rl);
+#pragma unroll
+      for (int t = 0; t < R; ++t) x[t][e] = A[off + 16u * t];
+    }
+  };
+  auto step = [&](int st, const double (&x)[R][8], int stg) {
+    v8s ah[R], am[R], al[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int kk = kBfK * st + 8 * lk + e;
+        unsigned short h, m, l;
+        split3(kk < K ? x[t][e] : 0.0, h, m, l);
+        ah[t][e] = static_cast<short>(h);
+        am[t][e] = static_cast<short>(m);
+        al[t][e] = static_cast<short>(l);
+      }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const v8s bh = __builtin_bit_cast(v8s, stage[stg][0][q][lane]);
+      const v8s bm = __builtin_bit_cast(v8s, stage[stg][1][q][lane]);
+      const v8s bl = __builtin_bit_cast(v8s, stage[stg][2][q][lane]);
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm, am[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, am[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm, ah[t], acc[t][q], 0, 0, 0);
+        acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah[t], acc[t][q], 0, 0, 0);
+      }
+    }
+  };
+  double x0[R][8], x1[R][8];
+  loadP(0);
+  writeP(0);
+  loadA(0, x0);
+  __syncthreads();
+  for (int st = 0; st < nst2; st += 2) {
+    // step st from stage 0 (loads of step st + 1 in flight)
+    loadP(st + 1);
+    loadA(st + 1, x1);
+    __builtin_amdgcn_sched_barrier(0);
+    if (!ghost && st < nsteps) step(st, x0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    writeP(1);
+    __syncthreads();
+    // step st + 1 from stage 1
+    loadP(st + 2);
+    loadA(st + 2, x0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (!ghost && st + 1 < nsteps) step(st + 1, x1, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    writeP(0);
+    __syncthreads();
+  }
+  // C += alpha * (FP32 block product) in FP64 (ghosts store nothing); lane l
+  // holds rows r0 + 16t + l%16 of columns 16q + 4(l/16) + v
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    double cv[4][R];
+    int64_t cb[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t e = ghost ? -1 : cbs[16 * q + 4 * lk + v];
+      cb[v] = ccol_off(e);
+      const double* cq = c_src(p, e, rl);
+#pragma unroll
+      for (int t = 0; t < R; ++t) cv[v][t] = cq[16 * t];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double* cq = C + (cb[v] >= 0 ? cb[v] : 0) + rl;
+#pragma unroll
+      for (int t = 0; t < R; ++t)
+        if (cb[v] >= 0 && rl + 16 * t < M) cq[16 * t] = cv[v][t] + p.alpha * static_cast<double>(acc[t][q][v]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Workgroups of four waves over four 32-row chunks of one block row (the
+// LDS-shared split-BF16 path).  Every wave runs the same sweep list, ghosts
+// included, so the workgroup barriers match.
+__global__ __launch_bounds__(256) SHT_ASP_OCC void k_asb(AsArgs p)
+{
+  constexpr int W = 4;
+  const int wave = uni(static_cast<int>(threadIdx.x >> 6));
+  const int64_t ui = static_cast<int64_t>(blockIdx.x) * W;
+  if (ui >= p.nunits) return;  // (the unit list is padded to whole workgroups)
+  const AsUnit u = p.units[ui + wave];
+  const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
+  const bool ghost = uni(u.pad) != 0;
+  if (nsw == 0) return;  // a whole-workgroup filler (every wave of it)
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  __shared__ int64_t cbs_all[W][128];
+  __shared__ v4u stage[2][3][kAsSweepSlots][64];
+  int64_t* cbs = cbs_all[wave];
+  SweepU cur = to_uni(p.sweeps[s0]);
+  SweepU nxt = to_uni(p.sweeps[s0 + min(1, nsw - 1)]);
+  AsSweep raw2 = p.sweeps[s0 + min(2, nsw - 1)];
+  int64_t cc0, cc1;
+  load_cols(p, cur, lane, cc0, cc1);
+  for (int j = 0; j < nsw; ++j) {
+    cbs[lane] = cc0;
+    cbs[lane + 64] = cc1;
+    wave_sync();
+    load_cols(p, nxt, lane, cc0, cc1);
+    switch (cur.ns) {
+      case 1: sweep_bf3_lds<1>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 2: sweep_bf3_lds<2>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 3: sweep_bf3_lds<3>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 4: sweep_bf3_lds<4>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 5: sweep_bf3_lds<5>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 6: sweep_bf3_lds<6>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 7: sweep_bf3_lds<7>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      default: sweep_bf3_lds<8>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+    }
+    wave_sync();
+    cur = nxt;
+    nxt = to_uni(raw2);
+    raw2 = p.sweeps[s0 + min(j + 3, nsw - 1)];
+  }
+}
+#endif  // SHT_AS_BF3_TU
+
 #ifndef SHT_AS_BF3_TU
 // ---------------------------------------------------------------------------
 // A-stationary fma chain (bitwise equal to the oracle).  One wave per 64-row
@@ -885,7 +1073,7 @@ void launch_as_bf3(AsClass k, const AsArgs& a, hipStream_t s)
   const dim3 gg(static_cast<unsigned>((a.nunits + G - 1) / G)), bg(64 * G);
   switch (k) {
     case AsClass::Bf16x3: hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Bf16x3G: hipLaunchKernelGGL((k_asp<2, 8, 2, G, false>), gg, bg, 0, s, a); break;
+    case AsClass::Bf16x3G: hipLaunchKernelGGL(k_asb, gg, bg, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "not a split-BF16 class");
   }
 }
