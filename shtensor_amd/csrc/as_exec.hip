@@ -792,7 +792,9 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
   const int64_t qstep = 16 * static_cast<int64_t>(kp);
   double* C = static_cast<double*>(p.C);
   const int K = sw.k, M = sw.m;
-  const int rl = r0 + li;
+  // lane li holds rows r0 + 2li (tile 0) and r0 + 2li + 1 (tile 1): one
+  // 16-byte load covers both tiles' rows of an A or C column
+  const int rl2 = r0 + 2 * li;
   const int nsteps = (K + kBfK - 1) / kBfK;
   const int nst2 = (nsteps + 1) & ~1;  // the loop runs whole pairs of steps
   v4f acc[R][NS];
@@ -819,14 +821,15 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) stage[stg][pl][wave + 4 * x][lane] = g[x][pl];
   };
-  // raw A of step st: rows rl, rl + 16, contracted indices 32st + 8lk + e
+  // raw A of step st: rows rl2, rl2 + 1, contracted indices 32st + 8lk + e
   auto loadA = [&](int st, double (&x)[R][8]) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int kk = kBfK * min(st, nsteps - 1) + 8 * lk + e;
-      const uint32_t off = ghost ? static_cast<uint32_t>(rl) : static_cast<uint32_t>(min(kk, K - 1) * M + rl);
-#pragma unroll
-      for (int t = 0; t < R; ++t) x[t][e] = A[off + 16u * t];
+      const uint32_t off = ghost ? static_cast<uint32_t>(rl2) : static_cast<uint32_t>(min(kk, K - 1) * M + rl2);
+      const v2d8 v = *reinterpret_cast<const v2d8*>(A + off);
+      x[0][e] = v.x;
+      x[1][e] = v.y;
     }
   };
   auto step = [&](int st, const double (&x)[R][8], int stg) {
@@ -882,7 +885,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
     __syncthreads();
   }
   // C += alpha * (FP32 block product) in FP64 (ghosts store nothing); lane l
-  // holds rows r0 + 16t + l%16 of columns 16q + 4(l/16) + v
+  // holds rows r0 + 2(l%16) + t of columns 16q + 4(l/16) + v
 #pragma unroll
   for (int q = 0; q < NS; ++q) {
     double cv[4][R];
@@ -891,16 +894,18 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
     for (int v = 0; v < 4; ++v) {
       const int64_t e = ghost ? -1 : cbs[16 * q + 4 * lk + v];
       cb[v] = ccol_off(e);
-      const double* cq = c_src(p, e, rl);
-#pragma unroll
-      for (int t = 0; t < R; ++t) cv[v][t] = cq[16 * t];
+      const v2d8 c2 = *reinterpret_cast<const v2d8*>(c_src(p, e, rl2));
+      cv[v][0] = c2.x;
+      cv[v][1] = c2.y;
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      double* cq = C + (cb[v] >= 0 ? cb[v] : 0) + rl;
-#pragma unroll
-      for (int t = 0; t < R; ++t)
-        if (cb[v] >= 0 && rl + 16 * t < M) cq[16 * t] = cv[v][t] + p.alpha * static_cast<double>(acc[t][q][v]);
+      double* cq = C + (cb[v] >= 0 ? cb[v] : 0) + rl2;
+      v2d8 c2;
+      c2.x = cv[v][0] + p.alpha * static_cast<double>(acc[0][q][v]);
+      c2.y = cv[v][1] + p.alpha * static_cast<double>(acc[1][q][v]);
+      if (cb[v] >= 0 && rl2 + 1 < M) *reinterpret_cast<v2d8*>(cq) = c2;
+      else if (cb[v] >= 0 && rl2 < M) cq[0] = c2.x;
     }
     __builtin_amdgcn_sched_barrier(0);
   }
