@@ -57,6 +57,7 @@ enum class KClass : int {
   Mfma64W,                           // FP64 MFMA, 64 x 64 tiles, four waves per tile
   F32W,                              // FP32 MFMA (FP64 cross-task), 64 x 64 tiles, four waves
   Mfma16q, Mfma32q,                  // FP64 MFMA on 128-row tiles (more than 96 rows remain)
+  Bf16W,                             // split-BF16 MFMA (FP64 cross-task), 64 x 64 tiles, four waves
   Count
 };
 

@@ -278,13 +278,13 @@ def test_autotune_keeps_results(ctx):
     plan.close()
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.5, -1.25)])
 def test_wide_blocks_c_stationary(ctx, alpha, beta, prec):
     """C blocks of 33..130 columns run on the four-wave 64 x 64 tiles (A and
     B chunks shared through LDS), with partial tiles in both directions and
     contracted lengths that are not multiples of 16: FP64 bit for bit equal
-    to the oracle's fma chain, FP32 within 1e-6."""
+    to the oracle's fma chain, FP32 and split-BF16 within 1e-6."""
     rng = np.random.default_rng(78)
     expr = "ik,kj->ij"
     bs_i = [16, 50, 64, 100, 7]
