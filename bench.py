@@ -43,7 +43,7 @@ KERNEL_SYMBOL = {
     "as_mfma32_n4": "k_asp<4, 4, 1, 1, false>",
     "as_exact": "k_as_exact", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
     "mfma64_n64w": "k_mfmaw<0>", "mfma32_n64w": "k_mfmaw<1>",
-    "mfma64_n16_r128": "k_mfma64<1, 8>", "mfma64_n32_r128": "k_mfma64<2, 8>", "bf16x3_n64w": "k_bf3w",
+    "mfma64_n16_r128": "k_mfma64<1, 8>", "mfma64_n32_r128": "k_mfma64<2, 8>", "bf16x3_n64w": "k_bf3w<64>", "bf16x3_n32w": "k_bf3w<32>",
 }
 for _name, _kern in (("mfma64", "k_mfma64"), ("mfma32", "k_mfma32"), ("bf16x3", "k_bf16x3")):
     for _ct, _n in ((1, 16), (2, 32)):

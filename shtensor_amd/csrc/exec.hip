@@ -19,8 +19,9 @@
 //  * k_mfmaw<F> — FP64 (F = 0) or FP32-per-task (F = 1) MFMA, four waves
 //    per 64 x 64 tile sharing every A and B chunk through LDS (C blocks
 //    more than 32 columns wide).
-//  * k_bf3w — split-BF16, four waves per 64 x 64 tile, A and B split once
-//    per chunk into bf16 planes in LDS (C blocks more than 32 columns wide).
+//  * k_bf3w<TS> — split-BF16, four waves per TS x TS tile (TS = 64: C
+//    blocks more than 32 columns wide; TS = 32: 17 to 32 columns), A and B
+//    split once per chunk into bf16 planes in LDS.
 //  RT = 4 (64-row tiles) in general; RT = 2 (32-row tiles) for a block's
 //  last row tile when at most 32 rows remain, so blocks of <= 32 rows (and
 //  the remainders of larger ones) issue no MFMAs on padding rows.
@@ -476,36 +477,40 @@ __device__ inline void split3_cs(double xd, short& h, short& m, short& l)
   l = static_cast<short>(bf16_bits_cs(r2));
 }
 
-// Split-BF16 on 64 x 64 C tiles (C blocks more than 32 columns wide), one
-// workgroup of four waves per tile.  Each 32-k chunk of A (64 rows x 32) and
-// B (32 x 64 columns) is loaded once by the workgroup (lane: one row or
-// column, eight consecutive k), split into three bf16 planes as it is
-// written to LDS (two stages, one barrier per chunk; rows padded to 40 bf16)
-// and read by the waves as 16-byte MFMA fragments; wave w owns rows
-// 32(w & 1) .. +31 and columns 32(w >> 1) .. +31.  The six products per
+// Split-BF16 on TS x TS C tiles, one workgroup of four waves per tile (TS =
+// 64 for C blocks more than 32 columns wide, 32 for 17 to 32 columns).  Each
+// 32-k chunk of A (TS rows x 32) and B (32 x TS columns) is loaded once by
+// the workgroup (lane: one row or column, TS/8 consecutive k), split into
+// three bf16 planes as it is written to LDS (two stages, one barrier per
+// chunk; rows padded to 40 bf16) and read by the waves as 16-byte MFMA
+// fragments; wave w owns rows (TS/2)(w & 1) .. and columns (TS/2)(w >> 1) ..
+// of the tile (TS/32 x TS/32 MFMA tiles).  The six products per
 // fragment pair, their order, FP32 per task and FP64 across tasks are those
 // of k_bf16x3.  Loads are unconditional (clamped in-bounds addresses); k >= K
 // is zeroed before the split.
 constexpr int kW3Pitch = 40;
+template <int TS>
 __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
 {
+  constexpr int T = TS / 32;   // 16 x 16 MFMA tiles per wave and direction
+  constexpr int KL = TS / 8;   // contracted indices per loader lane and chunk
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
   const CBlkRec cb = p.cblk[tile.cb];
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
+  const int wr = (TS / 2) * (w & 1), wc = (TS / 2) * (w >> 1);
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ short sA[2][3][64][kW3Pitch];  // [stage][plane][row of the tile][k]
-  __shared__ short sB[2][3][64][kW3Pitch];  // [stage][plane][column of the tile][k]
-  v4d acc[2][2];
+  __shared__ short sA[2][3][TS][kW3Pitch];  // [stage][plane][row of the tile][k]
+  __shared__ short sB[2][3][TS][kW3Pitch];  // [stage][plane][column of the tile][k]
+  v4d acc[T][T];
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < T; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -514,25 +519,25 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       }
   if (!p.alpha_zero && cb.t1 > cb.t0) {
     // loader lanes: row (of A) / column (of B) rc, contracted indices
-    // 8g .. 8g + 7 of the chunk
-    const int rc = tid & 63, g = tid >> 6;
+    // KL*g .. KL*g + KL-1 of the chunk
+    const int rc = tid % TS, g = tid / TS;
     const int ia = min(tile.m0 + rc, M - 1);
     const int jb = min(tile.n0 + rc, N - 1);
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
     TaskRec tr = p.tasks[tl];
-    double ra[8], rb[8];
+    double ra[KL], rb[KL];
     int k0l, Kl;  // contracted index of ra[0] / rb[0] and the task's K
     bool lastl;   // the loaded chunk is the last of its task (uniform)
 #define SHT_W3_LOAD()                                                                      \
   {                                                                                        \
     const int K = tr.k;                                                                    \
-    k0l = kl + 8 * g;                                                                      \
+    k0l = kl + KL * g;                                                                     \
     Kl = K;                                                                                \
     const double* pa = A + tr.a_off + ia;                                                  \
-    _Pragma("unroll") for (int e = 0; e < 8; ++e)                                          \
+    _Pragma("unroll") for (int e = 0; e < KL; ++e)                                         \
         ra[e] = pa[static_cast<int64_t>(min(k0l + e, K - 1)) * M];                         \
     const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(k0l, K - 1);      \
-    _Pragma("unroll") for (int e = 0; e < 8; e += 2) {                                     \
+    _Pragma("unroll") for (int e = 0; e < KL; e += 2) {                                    \
       const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                               \
       rb[e] = y.x;                                                                         \
       rb[e + 1] = y.y;                                                                     \
@@ -549,33 +554,34 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
   }
 #define SHT_W3_STORE(stg)                                                                  \
   {                                                                                        \
-    v8s h, m, l;                                                                           \
-    _Pragma("unroll") for (int e = 0; e < 8; ++e) {                                        \
+    typedef short vks __attribute__((ext_vector_type(KL)));                                \
+    vks h, m, l;                                                                           \
+    _Pragma("unroll") for (int e = 0; e < KL; ++e) {                                       \
       short x, y, z;                                                                       \
       split3_cs(k0l + e < Kl ? ra[e] : 0.0, x, y, z);                                      \
       h[e] = x;                                                                            \
       m[e] = y;                                                                            \
       l[e] = z;                                                                            \
     }                                                                                      \
-    *reinterpret_cast<v8s*>(&sA[stg][0][rc][8 * g]) = h;                                   \
-    *reinterpret_cast<v8s*>(&sA[stg][1][rc][8 * g]) = m;                                   \
-    *reinterpret_cast<v8s*>(&sA[stg][2][rc][8 * g]) = l;                                   \
-    _Pragma("unroll") for (int e = 0; e < 8; ++e) {                                        \
+    *reinterpret_cast<vks*>(&sA[stg][0][rc][KL * g]) = h;                                  \
+    *reinterpret_cast<vks*>(&sA[stg][1][rc][KL * g]) = m;                                  \
+    *reinterpret_cast<vks*>(&sA[stg][2][rc][KL * g]) = l;                                  \
+    _Pragma("unroll") for (int e = 0; e < KL; ++e) {                                       \
       short x, y, z;                                                                       \
       split3_cs(k0l + e < Kl ? rb[e] : 0.0, x, y, z);                                      \
       h[e] = x;                                                                            \
       m[e] = y;                                                                            \
       l[e] = z;                                                                            \
     }                                                                                      \
-    *reinterpret_cast<v8s*>(&sB[stg][0][rc][8 * g]) = h;                                   \
-    *reinterpret_cast<v8s*>(&sB[stg][1][rc][8 * g]) = m;                                   \
-    *reinterpret_cast<v8s*>(&sB[stg][2][rc][8 * g]) = l;                                   \
+    *reinterpret_cast<vks*>(&sB[stg][0][rc][KL * g]) = h;                                  \
+    *reinterpret_cast<vks*>(&sB[stg][1][rc][KL * g]) = m;                                  \
+    *reinterpret_cast<vks*>(&sB[stg][2][rc][KL * g]) = l;                                  \
   }
-    v4f part[2][2];
+    v4f part[T][T];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < T; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
     SHT_W3_LOAD()
     SHT_W3_STORE(0)
     bool lastc = lastl;  // the chunk in the current stage ends its task
@@ -585,23 +591,23 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       const bool more = tl < cb.t1;  // a next chunk exists (uniform)
       SHT_W3_LOAD()                  // (past the end: a clamped re-read, unused)
       {
-        v8s ah[2], am[2], al[2], bh[2], bm[2], bl[2];
+        v8s ah[T], am[T], al[T], bh[T], bm[T], bl[T];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) {
+        for (int rt = 0; rt < T; ++rt) {
           ah[rt] = *reinterpret_cast<const v8s*>(&sA[stg][0][wr + 16 * rt + li][8 * lk]);
           am[rt] = *reinterpret_cast<const v8s*>(&sA[stg][1][wr + 16 * rt + li][8 * lk]);
           al[rt] = *reinterpret_cast<const v8s*>(&sA[stg][2][wr + 16 * rt + li][8 * lk]);
         }
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < T; ++ct) {
           bh[ct] = *reinterpret_cast<const v8s*>(&sB[stg][0][wc + 16 * ct + li][8 * lk]);
           bm[ct] = *reinterpret_cast<const v8s*>(&sB[stg][1][wc + 16 * ct + li][8 * lk]);
           bl[ct] = *reinterpret_cast<const v8s*>(&sB[stg][2][wc + 16 * ct + li][8 * lk]);
         }
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
+          for (int ct = 0; ct < T; ++ct) {
             part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], al[rt], part[rt][ct], 0, 0, 0);
             part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm[ct], am[rt], part[rt][ct], 0, 0, 0);
             part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[ct], ah[rt], part[rt][ct], 0, 0, 0);
@@ -612,9 +618,9 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       }
       if (lastc) {  // the task's block product is complete: add it in FP64
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
+          for (int ct = 0; ct < T; ++ct) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
             part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -630,9 +636,9 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
 #undef SHT_W3_STORE
   }
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < T; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -785,7 +791,8 @@ int kclass_rows(KClass k)
     case KClass::F32x16h:
     case KClass::F32x32h:
     case KClass::Bf16x16h:
-    case KClass::Bf16x32h: return 32;
+    case KClass::Bf16x32h:
+    case KClass::Bf16S: return 32;
     case KClass::Mfma16q:
     case KClass::Mfma32q: return 128;
     default: return 64;  // (Mfma64W: 64)
@@ -808,6 +815,7 @@ int kclass_cols(KClass k)
     case KClass::Mfma64W:
     case KClass::F32W:
     case KClass::Bf16W: return 64;
+    case KClass::Bf16S: return 32;
     default: return 32;
   }
 }
@@ -836,7 +844,8 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
     case KClass::Mfma64W: hipLaunchKernelGGL(k_mfmaw<0>, g, dim3(4 * kWave), 0, s, a); break;
     case KClass::F32W: hipLaunchKernelGGL(k_mfmaw<1>, g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Bf16W: hipLaunchKernelGGL(k_bf3w, g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Bf16W: hipLaunchKernelGGL(k_bf3w<64>, g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Bf16S: hipLaunchKernelGGL(k_bf3w<32>, g, dim3(4 * kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
   SHT_HIP(hipGetLastError());

@@ -521,7 +521,7 @@ void build_schedule(sht_plan* P)
     KClass k;
     if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
     else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : (N <= 32 ? KClass::F32x32 : KClass::F32W);
-    else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : (N <= 32 ? KClass::Bf16x32 : KClass::Bf16W);
+    else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
     else k = N <= 16 ? KClass::Mfma16 : (N <= 32 ? KClass::Mfma32 : KClass::Mfma64W);
     const int cols = kclass_cols(k);
     const int64_t rk = row_key(P->C->indices[c]);
@@ -1092,7 +1092,7 @@ const char* sht_kclass_name(int k)
                                 "mfma64_n32",    "mfma32_n16",    "mfma32_n32",    "bf16x3_n16",
                                 "bf16x3_n32",    "mfma64_n16_r32", "mfma64_n32_r32", "mfma32_n16_r32",
                                 "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32", "mfma64_n64w", "mfma32_n64w", "mfma64_n16_r128",
-                                "mfma64_n32_r128", "bf16x3_n64w"};
+                                "mfma64_n32_r128", "bf16x3_n64w", "bf16x3_n32w"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
   return names[k];
 }
