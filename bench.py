@@ -42,8 +42,9 @@ KERNEL_SYMBOL = {
     "as_bf16x3_g": "k_asp<2, 8, 2, 4, false>", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true>",
     "as_mfma32_n4": "k_asp<4, 4, 1, 1, false>",
     "as_exact": "k_as_exact", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
-    "mfma64_n64w": "k_mfmaw<0>", "mfma32_n64w": "k_mfmaw<1>",
+    "mfma64_n64w": "k_mfmaw<0, 64>", "mfma32_n64w": "k_mfmaw<1, 64>",
     "mfma64_n16_r128": "k_mfma64<1, 8>", "mfma64_n32_r128": "k_mfma64<2, 8>", "bf16x3_n64w": "k_bf3w<64>", "bf16x3_n32w": "k_bf3w<32>",
+    "mfma64_n32s": "k_mfmaw<0, 32>", "mfma32_n32s": "k_mfmaw<1, 32>",
 }
 for _name, _kern in (("mfma64", "k_mfma64"), ("mfma32", "k_mfma32"), ("bf16x3", "k_bf16x3")):
     for _ct, _n in ((1, 16), (2, 32)):

@@ -228,9 +228,11 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 // after task in task order and k ascending, so every C element sees the
 // same chain as in the one-wave kernels of the same precision.
 // ---------------------------------------------------------------------------
-template <int F>
+template <int F, int TS>
 __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 {
+  constexpr int T = TS / 32;   // 16 x 16 MFMA tiles per wave and direction
+  constexpr int E = TS / 16;   // elements of A and of B per loader lane and chunk
   typedef typename std::conditional<F == 0, double, float>::type TL;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
@@ -238,19 +240,19 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
+  const int wr = (TS / 2) * (w & 1), wc = (TS / 2) * (w >> 1);
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ TL sA[2][16][64];  // [stage][k][row of the tile]
-  __shared__ TL sB[2][16][64];  // [stage][k][column of the tile]
+  __shared__ TL sA[2][16][TS];  // [stage][k][row of the tile]
+  __shared__ TL sB[2][16][TS];  // [stage][k][column of the tile]
   // column of accumulator element v (the f64 and f32 MFMA results differ)
   auto colof = [&](int ct, int v) { return tile.n0 + wc + 16 * ct + (F == 0 ? 4 * v + lk : 4 * lk + v); };
-  v4d acc[2][2];
+  v4d acc[T][T];
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < T; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -258,40 +260,44 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero && cb.t1 > cb.t0) {
-    // loader lanes: A column kA (0..15) rows rA..rA+3; B column jB (0..63)
-    // contracted rows kB..kB+3
-    const int kA = tid >> 4, rA = 4 * (tid & 15);
-    const int jB = tid >> 2, kB = 4 * (tid & 3);
+    // loader lanes: A column kA (0..15) rows rA..rA+E-1; B column jB
+    // (0..TS-1) contracted rows kB..kB+E-1
+    const int kA = tid >> 4, rA = E * (tid & 15);
+    const int jB = tid / (256 / TS), kB = E * (tid % (256 / TS));
     const int ia = min(tile.m0 + rA, M - 1);
     const int jb = min(tile.n0 + jB, N - 1);
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
     TaskRec tr = p.tasks[tl];
-    double ra[4], rb[4];
+    double ra[E], rb[E];
     bool ka;       // this lane's A column is inside the task
     int kbb, Kb;   // contracted index of rb[0] and the task's K
     bool lastl;    // the loaded chunk is the last of its task (uniform)
     const double alpha = p.alpha;
     const bool a1 = p.alpha_one != 0;
-    v4f part[2][2];
+    v4f part[T][T];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < T; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
 #define SHT_W_LOAD()                                                                       \
   {                                                                                        \
     const int K = tr.k;                                                                    \
     const int kk = kl + kA;                                                                \
     ka = kk < K;                                                                           \
     const double* pa = A + tr.a_off + static_cast<int64_t>(min(kk, K - 1)) * M + ia;       \
-    const v2d8 a01 = *reinterpret_cast<const v2d8*>(pa);                                   \
-    const v2d8 a23 = *reinterpret_cast<const v2d8*>(pa + 2);                               \
+    _Pragma("unroll") for (int e = 0; e < E; e += 2) {                                     \
+      const v2d8 x = *reinterpret_cast<const v2d8*>(pa + e);                               \
+      ra[e] = x.x;                                                                         \
+      ra[e + 1] = x.y;                                                                     \
+    }                                                                                      \
     kbb = kl + kB;                                                                         \
     Kb = K;                                                                                \
     const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(kbb, K - 1);      \
-    const v2d8 b01 = *reinterpret_cast<const v2d8*>(pb);                                   \
-    const v2d8 b23 = *reinterpret_cast<const v2d8*>(pb + 2);                               \
-    ra[0] = a01.x; ra[1] = a01.y; ra[2] = a23.x; ra[3] = a23.y;                            \
-    rb[0] = b01.x; rb[1] = b01.y; rb[2] = b23.x; rb[3] = b23.y;                            \
+    _Pragma("unroll") for (int e = 0; e < E; e += 2) {                                     \
+      const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                               \
+      rb[e] = y.x;                                                                         \
+      rb[e + 1] = y.y;                                                                     \
+    }                                                                                      \
     lastl = kl + 16 >= K;                                                                  \
     if (tl < cb.t1) {                                                                      \
       kl += 16;                                                                            \
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   }
 #define SHT_W_STORE(stg)                                                                   \
   {                                                                                        \
-    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                        \
+    _Pragma("unroll") for (int e = 0; e < E; ++e) {                                        \
       const double x = ka ? ra[e] : 0.0;                                                   \
       const double y = kbb + e < Kb ? rb[e] : 0.0;                                         \
       if constexpr (F == 0) {                                                              \
@@ -326,15 +332,15 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
       SHT_W_LOAD()                   // (past the end: a clamped re-read, unused)
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        TL a[2], b[2];
+        TL a[T], b[T];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) a[rt] = sA[stg][4 * s + lk][wr + 16 * rt + li];
+        for (int rt = 0; rt < T; ++rt) a[rt] = sA[stg][4 * s + lk][wr + 16 * rt + li];
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) b[ct] = sB[stg][4 * s + lk][wc + 16 * ct + li];
+        for (int ct = 0; ct < T; ++ct) b[ct] = sB[stg][4 * s + lk][wc + 16 * ct + li];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
+          for (int ct = 0; ct < T; ++ct) {
             if constexpr (F == 0)
               acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
             else
@@ -344,9 +350,9 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
       if constexpr (F == 1) {
         if (lastc) {  // the task's block product is complete: add it in FP64
 #pragma unroll
-          for (int rt = 0; rt < 2; ++rt)
+          for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
+            for (int ct = 0; ct < T; ++ct) {
 #pragma unroll
               for (int v = 0; v < 4; ++v) acc[rt][ct][v] += alpha * static_cast<double>(part[rt][ct][v]);
               part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -363,9 +369,9 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 #undef SHT_W_STORE
   }
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < T; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < T; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -792,7 +798,9 @@ int kclass_rows(KClass k)
     case KClass::F32x32h:
     case KClass::Bf16x16h:
     case KClass::Bf16x32h:
-    case KClass::Bf16S: return 32;
+    case KClass::Bf16S:
+    case KClass::Mfma64S:
+    case KClass::F32S: return 32;
     case KClass::Mfma16q:
     case KClass::Mfma32q: return 128;
     default: return 64;  // (Mfma64W: 64)
@@ -815,7 +823,9 @@ int kclass_cols(KClass k)
     case KClass::Mfma64W:
     case KClass::F32W:
     case KClass::Bf16W: return 64;
-    case KClass::Bf16S: return 32;
+    case KClass::Bf16S:
+    case KClass::Mfma64S:
+    case KClass::F32S: return 32;
     default: return 32;
   }
 }
@@ -842,8 +852,10 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Bf16x32h: hipLaunchKernelGGL((k_bf16x3<2, 2>), g, b, 0, s, a); break;
     case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
-    case KClass::Mfma64W: hipLaunchKernelGGL(k_mfmaw<0>, g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32W: hipLaunchKernelGGL(k_mfmaw<1>, g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::Bf16W: hipLaunchKernelGGL(k_bf3w<64>, g, dim3(4 * kWave), 0, s, a); break;
     case KClass::Bf16S: hipLaunchKernelGGL(k_bf3w<32>, g, dim3(4 * kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");

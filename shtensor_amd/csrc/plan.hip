@@ -432,9 +432,9 @@ KClass half_rows(KClass k)
 {
   switch (k) {
     case KClass::Mfma16: return KClass::Mfma16h;
-    case KClass::Mfma32: return KClass::Mfma32h;
+    case KClass::Mfma32: return KClass::Mfma64S;
     case KClass::F32x16: return KClass::F32x16h;
-    case KClass::F32x32: return KClass::F32x32h;
+    case KClass::F32x32: return KClass::F32S;
     case KClass::Bf16x16: return KClass::Bf16x16h;
     case KClass::Bf16x32: return KClass::Bf16x32h;
     default: return k;
@@ -1092,7 +1092,7 @@ const char* sht_kclass_name(int k)
                                 "mfma64_n32",    "mfma32_n16",    "mfma32_n32",    "bf16x3_n16",
                                 "bf16x3_n32",    "mfma64_n16_r32", "mfma64_n32_r32", "mfma32_n16_r32",
                                 "mfma32_n32_r32", "bf16x3_n16_r32", "bf16x3_n32_r32", "mfma64_n64w", "mfma32_n64w", "mfma64_n16_r128",
-                                "mfma64_n32_r128", "bf16x3_n64w", "bf16x3_n32w"};
+                                "mfma64_n32_r128", "bf16x3_n64w", "bf16x3_n32w", "mfma64_n32s", "mfma32_n32s"};
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
   return names[k];
 }

@@ -59,6 +59,7 @@ enum class KClass : int {
   Mfma16q, Mfma32q,                  // FP64 MFMA on 128-row tiles (more than 96 rows remain)
   Bf16W,                             // split-BF16 MFMA (FP64 cross-task), 64 x 64 tiles, four waves
   Bf16S,                             // split-BF16 MFMA, 32 x 32 tiles, four waves (17..32 columns)
+  Mfma64S, F32S,                     // FP64 / FP32 MFMA, 32 x 32 tiles, four waves
   Count
 };
 
