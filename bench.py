@@ -39,7 +39,7 @@ HBM_PEAK = 8.0e12
 KERNEL_SYMBOL = {
     "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asl<0, 4, true, 4>",
     "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asl<1, 4, false, 4>", "as_bf16x3": "k_asp<2, 8, 2, 1, false>",
-    "as_bf16x3_g": "k_asp<2, 8, 2, 4, false>", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true>",
+    "as_bf16x3_g": "k_asb", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true>",
     "as_mfma32_n4": "k_asp<4, 4, 1, 1, false>",
     "as_exact": "k_as_exact", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
     "mfma64_n64w": "k_mfmaw<0, 64>", "mfma32_n64w": "k_mfmaw<1, 64>",
