@@ -335,13 +335,15 @@ def test_large_launch_128_row_tiles(ctx):
     dC.close()
 
 
-@pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (1, 2, 3))])
+@pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (0, 1, 2, 3)),
+                                         (sht.PREC_BF16X3, (0, 1, 2))])
 def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
     """Block rows of 1..7 32-row chunks (so the grouped schedule pads its
     workgroups with 0..3 ghost units), panels of up to 17 16-column slots
     (so sweeps are cut at 8 and at 4 slots) and C blocks up to 70 columns wide
-    (C-stationary 64-column tiles plus a remainder): every schedule gives the
-    oracle's result, FP64 bit for bit."""
+    (C-stationary 64-column tiles plus a remainder, 32 x 32 four-wave tiles
+    for 17-32 columns): every schedule gives the oracle's result, FP64 bit
+    for bit, FP32 and split-BF16 within 1e-6."""
     rng = np.random.default_rng(77)
     expr = "ik,kj->ij"
     bs_i = [5, 32, 33, 64, 97, 130, 161, 200, 224, 17]
