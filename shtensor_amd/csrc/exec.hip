@@ -446,16 +446,32 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
           for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
     }
   }
+  // Write-back through LDS as in k_mfma64 (here lane (li, lk) holds column
+  // 4lk + v of each 16-column group): every store instruction writes whole
+  // column pieces of 64 x 8 bytes.
+  constexpr int R = 16 * RT;
+  constexpr int CPS = R >= 64 ? 1 : 64 / R;
+  constexpr int RPS = R >= 64 ? R / 64 : 1;
+  __shared__ double cst[4 * R];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
+  for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
+    for (int v = 0; v < 4; ++v) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + li;
-        const int j = tile.n0 + 16 * ct + 4 * lk + v;
-        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
-      }
+      for (int rt = 0; rt < RT; ++rt) cst[lk * R + 16 * rt + li] = acc[rt][ct][v];
+      wave_lds_sync();
+#pragma unroll
+      for (int c = 0; c < 4; c += CPS)
+#pragma unroll
+        for (int h = 0; h < RPS; ++h) {
+          const int cc = R >= 64 ? c : c + lane / R;
+          const int r = R >= 64 ? 64 * h + lane : lane % R;
+          const int i = tile.m0 + r;
+          const int j = tile.n0 + 16 * ct + 4 * cc + v;
+          if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[cc * R + r];
+        }
+      wave_lds_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -734,16 +750,32 @@ __global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
           for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
     }
   }
+  // Write-back through LDS as in k_mfma64 (here lane (li, lk) holds column
+  // 4lk + v of each 16-column group): every store instruction writes whole
+  // column pieces of 64 x 8 bytes.
+  constexpr int R = 16 * RT;
+  constexpr int CPS = R >= 64 ? 1 : 64 / R;
+  constexpr int RPS = R >= 64 ? R / 64 : 1;
+  __shared__ double cst[4 * R];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
+  for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
+    for (int v = 0; v < 4; ++v) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + li;
-        const int j = tile.n0 + 16 * ct + 4 * lk + v;
-        if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
-      }
+      for (int rt = 0; rt < RT; ++rt) cst[lk * R + 16 * rt + li] = acc[rt][ct][v];
+      wave_lds_sync();
+#pragma unroll
+      for (int c = 0; c < 4; c += CPS)
+#pragma unroll
+        for (int h = 0; h < RPS; ++h) {
+          const int cc = R >= 64 ? c : c + lane / R;
+          const int r = R >= 64 ? 64 * h + lane : lane % R;
+          const int i = tile.m0 + r;
+          const int j = tile.n0 + 16 * ct + 4 * cc + v;
+          if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[cc * R + r];
+        }
+      wave_lds_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
