@@ -825,17 +825,14 @@ int kclass_rows(KClass k)
     case KClass::Exact16:
     case KClass::Exact32: return 128;
     case KClass::Mfma16h:
-    case KClass::Mfma32h:
     case KClass::F32x16h:
-    case KClass::F32x32h:
     case KClass::Bf16x16h:
-    case KClass::Bf16x32h:
     case KClass::Bf16S:
     case KClass::Mfma64S:
     case KClass::F32S: return 32;
     case KClass::Mfma16q:
     case KClass::Mfma32q: return 128;
-    default: return 64;  // (Mfma64W: 64)
+    default: return 64;
   }
 }
 
@@ -875,13 +872,9 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::F32x16: hipLaunchKernelGGL((k_mfma32<1, 4>), g, b, 0, s, a); break;
     case KClass::F32x32: hipLaunchKernelGGL((k_mfma32<2, 4>), g, b, 0, s, a); break;
     case KClass::Bf16x16: hipLaunchKernelGGL((k_bf16x3<1, 4>), g, b, 0, s, a); break;
-    case KClass::Bf16x32: hipLaunchKernelGGL((k_bf16x3<2, 4>), g, b, 0, s, a); break;
     case KClass::Mfma16h: hipLaunchKernelGGL((k_mfma64<1, 2>), g, b, 0, s, a); break;
-    case KClass::Mfma32h: hipLaunchKernelGGL((k_mfma64<2, 2>), g, b, 0, s, a); break;
     case KClass::F32x16h: hipLaunchKernelGGL((k_mfma32<1, 2>), g, b, 0, s, a); break;
-    case KClass::F32x32h: hipLaunchKernelGGL((k_mfma32<2, 2>), g, b, 0, s, a); break;
     case KClass::Bf16x16h: hipLaunchKernelGGL((k_bf16x3<1, 2>), g, b, 0, s, a); break;
-    case KClass::Bf16x32h: hipLaunchKernelGGL((k_bf16x3<2, 2>), g, b, 0, s, a); break;
     case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
     case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64>), g, dim3(4 * kWave), 0, s, a); break;

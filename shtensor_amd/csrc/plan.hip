@@ -436,7 +436,6 @@ KClass half_rows(KClass k)
     case KClass::F32x16: return KClass::F32x16h;
     case KClass::F32x32: return KClass::F32S;
     case KClass::Bf16x16: return KClass::Bf16x16h;
-    case KClass::Bf16x32: return KClass::Bf16x32h;
     default: return k;
   }
 }
@@ -1091,12 +1090,11 @@ const char* sht_kclass_name(int k)
   static_assert(sizeof(as_names) / sizeof(as_names[0]) == static_cast<size_t>(sht::AsClass::Count),
                 "a name per A-stationary class");
   if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
-  static const char* names[] = {"exact_n8",        "exact_n16",       "exact_n32",      "mfma64_n16",
-                                "mfma64_n32",      "mfma32_n16",      "mfma32_n32",     "bf16x3_n16",
-                                "bf16x3_n32",      "mfma64_n16_r32",  "mfma64_n32_r32", "mfma32_n16_r32",
-                                "mfma32_n32_r32",  "bf16x3_n16_r32",  "bf16x3_n32_r32", "mfma64_n64w",
-                                "mfma32_n64w",     "mfma64_n16_r128", "mfma64_n32_r128", "bf16x3_n64w",
-                                "bf16x3_n32w",     "mfma64_n32s",     "mfma32_n32s"};
+  static const char* names[] = {"exact_n8",       "exact_n16",      "exact_n32",      "mfma64_n16",
+                                "mfma64_n32",     "mfma32_n16",     "mfma32_n32",     "bf16x3_n16",
+                                "mfma64_n16_r32", "mfma32_n16_r32", "bf16x3_n16_r32", "mfma64_n64w",
+                                "mfma32_n64w",    "bf16x3_n64w",    "mfma64_n32s",    "mfma32_n32s",
+                                "bf16x3_n32s",    "mfma64_n16_r128", "mfma64_n32_r128"};
   static_assert(sizeof(names) / sizeof(names[0]) == static_cast<size_t>(sht::KClass::Count),
                 "a name per C-stationary class");
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";

@@ -48,18 +48,17 @@ struct ExecArgs {
 // kernel classes of the C-stationary launches
 enum class KClass : int {
   Exact8 = 0, Exact16, Exact32,      // fma chain, 128 rows x {8,16,32} cols
-  Mfma16, Mfma32,                    // FP64 MFMA, 64 rows x {16,32} cols
-  F32x16, F32x32,                    // FP32 MFMA, FP64 cross-task
-  Bf16x16, Bf16x32,                  // split-BF16x3 MFMA, FP64 cross-task
-  Mfma16h, Mfma32h,                  // the three MFMA forms above on 32-row
-  F32x16h, F32x32h,                  // tiles (a block's last row tile when
-  Bf16x16h, Bf16x32h,                // at most 32 rows remain)
-  Mfma64W,                           // FP64 MFMA, 64 x 64 tiles, four waves per tile
-  F32W,                              // FP32 MFMA (FP64 cross-task), 64 x 64 tiles, four waves
-  Mfma16q, Mfma32q,                  // FP64 MFMA on 128-row tiles (more than 96 rows remain)
-  Bf16W,                             // split-BF16 MFMA (FP64 cross-task), 64 x 64 tiles, four waves
-  Bf16S,                             // split-BF16 MFMA, 32 x 32 tiles, four waves (17..32 columns)
-  Mfma64S, F32S,                     // FP64 / FP32 MFMA, 32 x 32 tiles, four waves
+  Mfma16, Mfma32,                    // FP64 MFMA, one wave, 64 rows x {16,32} cols
+  F32x16, F32x32,                    // FP32 MFMA (FP64 cross-task), one wave, 64 rows x {16,32} cols
+  Bf16x16,                           // split-BF16 MFMA (FP64 cross-task), one wave, 64 x 16
+  Mfma16h, F32x16h, Bf16x16h,        // the 16-column one-wave forms on 32-row tiles (a
+                                     // block's last row tile when at most 32 rows remain)
+  Mfma64W, F32W, Bf16W,              // four waves per 64 x 64 tile (more than 32 columns)
+  Mfma64S, F32S, Bf16S,              // four waves per 32 x 32 tile (17-32 columns: FP64 / FP32
+                                     // for a block's last row tile of at most 32 rows,
+                                     // split-BF16 for every row tile)
+  Mfma16q, Mfma32q,                  // FP64 one-wave 128-row tiles (more than 96 rows
+                                     // remain, large launches)
   Count
 };
 
