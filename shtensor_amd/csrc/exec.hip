@@ -13,18 +13,18 @@
 //    wave, accumulators persist across tasks (FP64 throughout).
 //  * k_mfma32<CT,RT> — v_mfma_f32_16x16x4f32 per task, task partial added
 //    into an FP64 accumulator (FP64 across blocks).
-//  * k_bf16x3<CT,RT> — three-term split-BF16 (x = hi + mid + lo), six
+//  * k_mfmaw<F,TS> — FP64 (F = 0) or FP32-per-task (F = 1) MFMA, four waves
+//    per TS x TS tile sharing every A and B chunk through LDS (TS = 64: C
+//    blocks more than 32 columns wide; TS = 32: a block's last row tile of
+//    at most 32 rows and 17 to 32 columns).
+//  * k_bf3w<TR,TC> — three-term split-BF16 (x = hi + mid + lo), six
 //    products of order <= 2 with v_mfma_f32_16x16x32_bf16, FP32 per task,
-//    FP64 across.
-//  * k_mfmaw<F> — FP64 (F = 0) or FP32-per-task (F = 1) MFMA, four waves
-//    per 64 x 64 tile sharing every A and B chunk through LDS (C blocks
-//    more than 32 columns wide).
-//  * k_bf3w<TS> — split-BF16, four waves per TS x TS tile (TS = 64: C
-//    blocks more than 32 columns wide; TS = 32: 17 to 32 columns), A and B
-//    split once per chunk into bf16 planes in LDS.
-//  RT = 4 (64-row tiles) in general; RT = 2 (32-row tiles) for a block's
-//  last row tile when at most 32 rows remain, so blocks of <= 32 rows (and
-//  the remainders of larger ones) issue no MFMAs on padding rows.
+//    FP64 across; four waves per TR x TC tile (64 x 64: C blocks more than
+//    32 columns wide; 32 x 32: 17 to 32 columns; 64 x 16: at most 16
+//    columns), A and B split once per chunk into bf16 planes in LDS.
+//  RT = 4 (64-row tiles) in general, RT = 8 while more than 96 rows remain
+//  in large FP64 launches, RT = 2 (32-row tiles) for a block's last row
+//  tile when at most 32 rows remain (no MFMAs on padding rows).
 
 #include <type_traits>
 
@@ -478,8 +478,9 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
 // Three-term split-BF16 (C-stationary).  v_mfma_f32_16x16x32_bf16 (wave64):
 // A frag lane l holds A[l%16][8*(l/16) .. 8*(l/16)+7]; B frag holds
 // B[8*(l/16)..+7][l%16]; D as the f32 form above.  x = hi + mid + lo (bf16
-// each, split3 below); the six products of order <= 2 accumulate in FP32
-// per task (smallest first), FP64 across tasks.
+// each, split3_cs below); the six products of order <= 2 accumulate in
+// FP32 per task (smallest first: lo*hi, mid*mid, hi*lo, mid*hi, hi*mid,
+// hi*hi), FP64 across tasks.
 // ---------------------------------------------------------------------------
 __device__ inline unsigned short bf16_bits_cs(float x)
 {
@@ -499,40 +500,42 @@ __device__ inline void split3_cs(double xd, short& h, short& m, short& l)
   l = static_cast<short>(bf16_bits_cs(r2));
 }
 
-// Split-BF16 on TS x TS C tiles, one workgroup of four waves per tile (TS =
-// 64 for C blocks more than 32 columns wide, 32 for 17 to 32 columns).  Each
-// 32-k chunk of A (TS rows x 32) and B (32 x TS columns) is loaded once by
-// the workgroup (lane: one row or column, TS/8 consecutive k), split into
-// three bf16 planes as it is written to LDS (two stages, one barrier per
-// chunk; rows padded to 40 bf16) and read by the waves as 16-byte MFMA
-// fragments; wave w owns rows (TS/2)(w & 1) .. and columns (TS/2)(w >> 1) ..
-// of the tile (TS/32 x TS/32 MFMA tiles).  The six products per
-// fragment pair, their order, FP32 per task and FP64 across tasks are those
-// of k_bf16x3.  Loads are unconditional (clamped in-bounds addresses); k >= K
-// is zeroed before the split.
+// Split-BF16 on TR x TC C tiles, one workgroup of four waves per tile (64 x
+// 64 for C blocks more than 32 columns wide, 32 x 32 for 17 to 32 columns,
+// 64 x 16 for at most 16).  Each 32-k chunk of A (TR rows x 32) and B (32 x
+// TC columns) is loaded once by the workgroup (lane: one row or column,
+// TR/8 or TC/8 consecutive k), split into three bf16 planes as it is
+// written to LDS (two stages, one barrier per chunk; rows padded to 40
+// bf16) and read by the waves as 16-byte MFMA fragments; the waves split
+// the tile into 2 x 2 (or 4 x 1 for 16 columns) parts of 16 x 16 MFMA
+// tiles.  Loads are unconditional (clamped in-bounds addresses); k >= K is
+// zeroed before the split.
 constexpr int kW3Pitch = 40;
-template <int TS>
+template <int TR, int TC>
 __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
 {
-  constexpr int T = TS / 32;   // 16 x 16 MFMA tiles per wave and direction
-  constexpr int KL = TS / 8;   // contracted indices per loader lane and chunk
+  // waves: WR along the rows x WC along the columns of the tile
+  constexpr int WC = TC >= 32 ? 2 : 1, WR = 4 / WC;
+  constexpr int TRt = TR / WR / 16, TCt = TC / WC / 16;  // 16 x 16 MFMA tiles per wave
+  constexpr int KA = TR / 8, KB = TC / 8;  // contracted indices per loader lane and chunk
+  static_assert(TRt >= 1 && TCt >= 1 && KB >= 2, "tile shape");
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
   const CBlkRec cb = p.cblk[tile.cb];
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int wr = (TS / 2) * (w & 1), wc = (TS / 2) * (w >> 1);
+  const int wr = (TR / WR) * (w % WR), wc = (TC / WC) * (w / WR);
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ short sA[2][3][TS][kW3Pitch];  // [stage][plane][row of the tile][k]
-  __shared__ short sB[2][3][TS][kW3Pitch];  // [stage][plane][column of the tile][k]
-  v4d acc[T][T];
+  __shared__ short sA[2][3][TR][kW3Pitch];  // [stage][plane][row of the tile][k]
+  __shared__ short sB[2][3][TC][kW3Pitch];  // [stage][plane][column of the tile][k]
+  v4d acc[TRt][TCt];
 #pragma unroll
-  for (int rt = 0; rt < T; ++rt)
+  for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < T; ++ct)
+    for (int ct = 0; ct < TCt; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -540,26 +543,28 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero && cb.t1 > cb.t0) {
-    // loader lanes: row (of A) / column (of B) rc, contracted indices
-    // KL*g .. KL*g + KL-1 of the chunk
-    const int rc = tid % TS, g = tid / TS;
-    const int ia = min(tile.m0 + rc, M - 1);
-    const int jb = min(tile.n0 + rc, N - 1);
+    // loader lanes: row ra_ of A with contracted indices KA*ga .. +KA-1 of
+    // the chunk; column cb_ of B with KB*gb .. +KB-1
+    const int ra_ = tid % TR, ga = tid / TR;
+    const int cb_ = tid % TC, gb = tid / TC;
+    const int ia = min(tile.m0 + ra_, M - 1);
+    const int jb = min(tile.n0 + cb_, N - 1);
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
     TaskRec tr = p.tasks[tl];
-    double ra[KL], rb[KL];
-    int k0l, Kl;  // contracted index of ra[0] / rb[0] and the task's K
+    double ra[KA], rb[KB];
+    int k0a, k0b, Kl;  // contracted index of ra[0] and of rb[0], the task's K
     bool lastl;   // the loaded chunk is the last of its task (uniform)
 #define SHT_W3_LOAD()                                                                      \
   {                                                                                        \
     const int K = tr.k;                                                                    \
-    k0l = kl + KL * g;                                                                     \
+    k0a = kl + KA * ga;                                                                    \
+    k0b = kl + KB * gb;                                                                    \
     Kl = K;                                                                                \
     const double* pa = A + tr.a_off + ia;                                                  \
-    _Pragma("unroll") for (int e = 0; e < KL; ++e)                                         \
-        ra[e] = pa[static_cast<int64_t>(min(k0l + e, K - 1)) * M];                         \
-    const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(k0l, K - 1);      \
-    _Pragma("unroll") for (int e = 0; e < KL; e += 2) {                                    \
+    _Pragma("unroll") for (int e = 0; e < KA; ++e)                                         \
+        ra[e] = pa[static_cast<int64_t>(min(k0a + e, K - 1)) * M];                         \
+    const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(k0b, K - 1);      \
+    _Pragma("unroll") for (int e = 0; e < KB; e += 2) {                                    \
       const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                               \
       rb[e] = y.x;                                                                         \
       rb[e + 1] = y.y;                                                                     \
@@ -576,34 +581,36 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
   }
 #define SHT_W3_STORE(stg)                                                                  \
   {                                                                                        \
-    typedef short vks __attribute__((ext_vector_type(KL)));                                \
-    vks h, m, l;                                                                           \
-    _Pragma("unroll") for (int e = 0; e < KL; ++e) {                                       \
+    typedef short vas __attribute__((ext_vector_type(KA)));                                \
+    typedef short vbs __attribute__((ext_vector_type(KB)));                                \
+    vas h, m, l;                                                                           \
+    _Pragma("unroll") for (int e = 0; e < KA; ++e) {                                       \
       short x, y, z;                                                                       \
-      split3_cs(k0l + e < Kl ? ra[e] : 0.0, x, y, z);                                      \
+      split3_cs(k0a + e < Kl ? ra[e] : 0.0, x, y, z);                                      \
       h[e] = x;                                                                            \
       m[e] = y;                                                                            \
       l[e] = z;                                                                            \
     }                                                                                      \
-    *reinterpret_cast<vks*>(&sA[stg][0][rc][KL * g]) = h;                                  \
-    *reinterpret_cast<vks*>(&sA[stg][1][rc][KL * g]) = m;                                  \
-    *reinterpret_cast<vks*>(&sA[stg][2][rc][KL * g]) = l;                                  \
-    _Pragma("unroll") for (int e = 0; e < KL; ++e) {                                       \
+    *reinterpret_cast<vas*>(&sA[stg][0][ra_][KA * ga]) = h;                                \
+    *reinterpret_cast<vas*>(&sA[stg][1][ra_][KA * ga]) = m;                                \
+    *reinterpret_cast<vas*>(&sA[stg][2][ra_][KA * ga]) = l;                                \
+    vbs hb, mb, lb;                                                                        \
+    _Pragma("unroll") for (int e = 0; e < KB; ++e) {                                       \
       short x, y, z;                                                                       \
-      split3_cs(k0l + e < Kl ? rb[e] : 0.0, x, y, z);                                      \
-      h[e] = x;                                                                            \
-      m[e] = y;                                                                            \
-      l[e] = z;                                                                            \
+      split3_cs(k0b + e < Kl ? rb[e] : 0.0, x, y, z);                                      \
+      hb[e] = x;                                                                           \
+      mb[e] = y;                                                                           \
+      lb[e] = z;                                                                           \
     }                                                                                      \
-    *reinterpret_cast<vks*>(&sB[stg][0][rc][KL * g]) = h;                                  \
-    *reinterpret_cast<vks*>(&sB[stg][1][rc][KL * g]) = m;                                  \
-    *reinterpret_cast<vks*>(&sB[stg][2][rc][KL * g]) = l;                                  \
+    *reinterpret_cast<vbs*>(&sB[stg][0][cb_][KB * gb]) = hb;                               \
+    *reinterpret_cast<vbs*>(&sB[stg][1][cb_][KB * gb]) = mb;                               \
+    *reinterpret_cast<vbs*>(&sB[stg][2][cb_][KB * gb]) = lb;                               \
   }
-    v4f part[T][T];
+    v4f part[TRt][TCt];
 #pragma unroll
-    for (int rt = 0; rt < T; ++rt)
+    for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < T; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < TCt; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
     SHT_W3_LOAD()
     SHT_W3_STORE(0)
     bool lastc = lastl;  // the chunk in the current stage ends its task
@@ -613,23 +620,23 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       const bool more = tl < cb.t1;  // a next chunk exists (uniform)
       SHT_W3_LOAD()                  // (past the end: a clamped re-read, unused)
       {
-        v8s ah[T], am[T], al[T], bh[T], bm[T], bl[T];
+        v8s ah[TRt], am[TRt], al[TRt], bh[TCt], bm[TCt], bl[TCt];
 #pragma unroll
-        for (int rt = 0; rt < T; ++rt) {
+        for (int rt = 0; rt < TRt; ++rt) {
           ah[rt] = *reinterpret_cast<const v8s*>(&sA[stg][0][wr + 16 * rt + li][8 * lk]);
           am[rt] = *reinterpret_cast<const v8s*>(&sA[stg][1][wr + 16 * rt + li][8 * lk]);
           al[rt] = *reinterpret_cast<const v8s*>(&sA[stg][2][wr + 16 * rt + li][8 * lk]);
         }
 #pragma unroll
-        for (int ct = 0; ct < T; ++ct) {
+        for (int ct = 0; ct < TCt; ++ct) {
           bh[ct] = *reinterpret_cast<const v8s*>(&sB[stg][0][wc + 16 * ct + li][8 * lk]);
           bm[ct] = *reinterpret_cast<const v8s*>(&sB[stg][1][wc + 16 * ct + li][8 * lk]);
           bl[ct] = *reinterpret_cast<const v8s*>(&sB[stg][2][wc + 16 * ct + li][8 * lk]);
         }
 #pragma unroll
-        for (int rt = 0; rt < T; ++rt)
+        for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < T; ++ct) {
+          for (int ct = 0; ct < TCt; ++ct) {
             part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], al[rt], part[rt][ct], 0, 0, 0);
             part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm[ct], am[rt], part[rt][ct], 0, 0, 0);
             part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[ct], ah[rt], part[rt][ct], 0, 0, 0);
@@ -640,9 +647,9 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       }
       if (lastc) {  // the task's block product is complete: add it in FP64
 #pragma unroll
-        for (int rt = 0; rt < T; ++rt)
+        for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < T; ++ct) {
+          for (int ct = 0; ct < TCt; ++ct) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
             part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -658,124 +665,15 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
 #undef SHT_W3_STORE
   }
 #pragma unroll
-  for (int rt = 0; rt < T; ++rt)
+  for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < T; ++ct)
+    for (int ct = 0; ct < TCt; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
         const int j = tile.n0 + wc + 16 * ct + 4 * lk + v;
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
-}
-
-template <int CT, int RT>
-__global__ __launch_bounds__(64) void k_bf16x3(ExecArgs p)
-{
-  const TileRec tile = p.tiles[blockIdx.x];
-  if (tile.pad) return;  // a padding tile of the XCD-interleaved order
-  const CBlkRec cb = p.cblk[tile.cb];
-  const int M = cb.m, N = cb.n;
-  const int lane = threadIdx.x;
-  const int li = lane & 15, lk = lane >> 4;
-  const double* A = static_cast<const double*>(p.A);
-  const double* B = static_cast<const double*>(p.B);
-  double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  v4d acc[RT][CT];
-#pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = tile.m0 + 16 * rt + li;
-        const int j = tile.n0 + 16 * ct + 4 * lk + v;
-        acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
-      }
-  if (!p.alpha_zero) {
-    for (int t = cb.t0; t < cb.t1; ++t) {
-      const TaskRec tr = p.tasks[t];
-      const double* Ab = A + tr.a_off;
-      const double* Bb = B + tr.b_off;
-      const int K = tr.k;
-      v4f part[RT][CT];
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
-      for (int k = 0; k < K; k += 32) {
-        v8s bh[CT], bm[CT], bl[CT];
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          const int j = tile.n0 + 16 * ct + li;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int kk = k + 8 * lk + e;
-            short h = 0, m = 0, l = 0;
-            if (kk < K && j < N) split3_cs(Bb[kk + static_cast<int64_t>(j) * K], h, m, l);
-            bh[ct][e] = h;
-            bm[ct][e] = m;
-            bl[ct][e] = l;
-          }
-        }
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          const int i = tile.m0 + 16 * rt + li;
-          v8s ah, am, al;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int kk = k + 8 * lk + e;
-            short h = 0, m = 0, l = 0;
-            if (kk < K && i < M) split3_cs(Ab[i + static_cast<int64_t>(kk) * M], h, m, l);
-            ah[e] = h;
-            am[e] = m;
-            al[e] = l;
-          }
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct) {
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], al, part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm[ct], am, part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[ct], ah, part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], am, part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm[ct], ah, part[rt][ct], 0, 0, 0);
-            part[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[ct], ah, part[rt][ct], 0, 0, 0);
-          }
-        }
-      }
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[rt][ct][v] += p.alpha * static_cast<double>(part[rt][ct][v]);
-    }
-  }
-  // Write-back through LDS as in k_mfma64 (here lane (li, lk) holds column
-  // 4lk + v of each 16-column group): every store instruction writes whole
-  // column pieces of 64 x 8 bytes.
-  constexpr int R = 16 * RT;
-  constexpr int CPS = R >= 64 ? 1 : 64 / R;
-  constexpr int RPS = R >= 64 ? R / 64 : 1;
-  __shared__ double cst[4 * R];
-#pragma unroll
-  for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt) cst[lk * R + 16 * rt + li] = acc[rt][ct][v];
-      wave_lds_sync();
-#pragma unroll
-      for (int c = 0; c < 4; c += CPS)
-#pragma unroll
-        for (int h = 0; h < RPS; ++h) {
-          const int cc = R >= 64 ? c : c + lane / R;
-          const int r = R >= 64 ? 64 * h + lane : lane % R;
-          const int i = tile.m0 + r;
-          const int j = tile.n0 + 16 * ct + 4 * cc + v;
-          if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = cst[cc * R + r];
-        }
-      wave_lds_sync();
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -826,7 +724,6 @@ int kclass_rows(KClass k)
     case KClass::Exact32: return 128;
     case KClass::Mfma16h:
     case KClass::F32x16h:
-    case KClass::Bf16x16h:
     case KClass::Bf16S:
     case KClass::Mfma64S:
     case KClass::F32S: return 32;
@@ -844,11 +741,10 @@ int kclass_cols(KClass k)
     case KClass::Exact32: return 32;
     case KClass::Mfma16:
     case KClass::F32x16:
-    case KClass::Bf16x16:
     case KClass::Mfma16h:
     case KClass::F32x16h:
-    case KClass::Bf16x16h:
-    case KClass::Mfma16q: return 16;
+    case KClass::Mfma16q:
+    case KClass::Bf16N: return 16;
     case KClass::Mfma64W:
     case KClass::F32W:
     case KClass::Bf16W: return 64;
@@ -871,18 +767,17 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Mfma32: hipLaunchKernelGGL((k_mfma64<2, 4>), g, b, 0, s, a); break;
     case KClass::F32x16: hipLaunchKernelGGL((k_mfma32<1, 4>), g, b, 0, s, a); break;
     case KClass::F32x32: hipLaunchKernelGGL((k_mfma32<2, 4>), g, b, 0, s, a); break;
-    case KClass::Bf16x16: hipLaunchKernelGGL((k_bf16x3<1, 4>), g, b, 0, s, a); break;
     case KClass::Mfma16h: hipLaunchKernelGGL((k_mfma64<1, 2>), g, b, 0, s, a); break;
     case KClass::F32x16h: hipLaunchKernelGGL((k_mfma32<1, 2>), g, b, 0, s, a); break;
-    case KClass::Bf16x16h: hipLaunchKernelGGL((k_bf16x3<1, 2>), g, b, 0, s, a); break;
     case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
     case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Bf16W: hipLaunchKernelGGL(k_bf3w<64>, g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Bf16S: hipLaunchKernelGGL(k_bf3w<32>, g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Bf16W: hipLaunchKernelGGL((k_bf3w<64, 64>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Bf16S: hipLaunchKernelGGL((k_bf3w<32, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Bf16N: hipLaunchKernelGGL((k_bf3w<64, 16>), g, dim3(4 * kWave), 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
   SHT_HIP(hipGetLastError());

@@ -435,7 +435,6 @@ KClass half_rows(KClass k)
     case KClass::Mfma32: return KClass::Mfma64S;
     case KClass::F32x16: return KClass::F32x16h;
     case KClass::F32x32: return KClass::F32S;
-    case KClass::Bf16x16: return KClass::Bf16x16h;
     default: return k;
   }
 }
@@ -520,7 +519,7 @@ void build_schedule(sht_plan* P)
     KClass k;
     if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
     else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : (N <= 32 ? KClass::F32x32 : KClass::F32W);
-    else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16x16 : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
+    else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16N : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
     else k = N <= 16 ? KClass::Mfma16 : (N <= 32 ? KClass::Mfma32 : KClass::Mfma64W);
     const int cols = kclass_cols(k);
     const int64_t rk = row_key(P->C->indices[c]);
@@ -1091,10 +1090,10 @@ const char* sht_kclass_name(int k)
                 "a name per A-stationary class");
   if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
   static const char* names[] = {"exact_n8",       "exact_n16",      "exact_n32",      "mfma64_n16",
-                                "mfma64_n32",     "mfma32_n16",     "mfma32_n32",     "bf16x3_n16",
-                                "mfma64_n16_r32", "mfma32_n16_r32", "bf16x3_n16_r32", "mfma64_n64w",
-                                "mfma32_n64w",    "bf16x3_n64w",    "mfma64_n32s",    "mfma32_n32s",
-                                "bf16x3_n32s",    "mfma64_n16_r128", "mfma64_n32_r128"};
+                                "mfma64_n32",     "mfma32_n16",     "mfma32_n32",     "mfma64_n16_r32",
+                                "mfma32_n16_r32", "mfma64_n64w",    "mfma32_n64w",    "bf16x3_n64w",
+                                "mfma64_n32s",    "mfma32_n32s",    "bf16x3_n32s",    "mfma64_n16_r128",
+                                "mfma64_n32_r128", "bf16x3_n16w"};
   static_assert(sizeof(names) / sizeof(names[0]) == static_cast<size_t>(sht::KClass::Count),
                 "a name per C-stationary class");
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";

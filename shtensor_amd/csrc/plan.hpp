@@ -50,8 +50,7 @@ enum class KClass : int {
   Exact8 = 0, Exact16, Exact32,      // fma chain, 128 rows x {8,16,32} cols
   Mfma16, Mfma32,                    // FP64 MFMA, one wave, 64 rows x {16,32} cols
   F32x16, F32x32,                    // FP32 MFMA (FP64 cross-task), one wave, 64 rows x {16,32} cols
-  Bf16x16,                           // split-BF16 MFMA (FP64 cross-task), one wave, 64 x 16
-  Mfma16h, F32x16h, Bf16x16h,        // the 16-column one-wave forms on 32-row tiles (a
+  Mfma16h, F32x16h,                  // the 16-column one-wave forms on 32-row tiles (a
                                      // block's last row tile when at most 32 rows remain)
   Mfma64W, F32W, Bf16W,              // four waves per 64 x 64 tile (more than 32 columns)
   Mfma64S, F32S, Bf16S,              // four waves per 32 x 32 tile (17-32 columns: FP64 / FP32
@@ -59,6 +58,7 @@ enum class KClass : int {
                                      // split-BF16 for every row tile)
   Mfma16q, Mfma32q,                  // FP64 one-wave 128-row tiles (more than 96 rows
                                      // remain, large launches)
+  Bf16N,                             // split-BF16, four waves per 64 x 16 tile (at most 16 columns)
   Count
 };
 
