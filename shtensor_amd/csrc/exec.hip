@@ -13,10 +13,12 @@
 //    wave, accumulators persist across tasks (FP64 throughout).
 //  * k_mfma32<CT,RT> — v_mfma_f32_16x16x4f32 per task, task partial added
 //    into an FP64 accumulator (FP64 across blocks).
-//  * k_mfmaw<F,TS> — FP64 (F = 0) or FP32-per-task (F = 1) MFMA, four waves
-//    per TS x TS tile sharing every A and B chunk through LDS (TS = 64: C
-//    blocks more than 32 columns wide; TS = 32: a block's last row tile of
-//    at most 32 rows and 17 to 32 columns).
+//  * k_mfmaw<F,TR,TC> — FP64 (F = 0) or FP32-per-task (F = 1) MFMA, four
+//    waves per TR x TC tile sharing every A and B chunk through LDS (64 x 64:
+//    C blocks more than 32 columns wide; 32 x 32: a block's last row tile of
+//    at most 32 rows and 17 to 32 columns; 64 x 16 and 64 x 32: at most 16
+//    and 17 to 32 columns; FP32 128 x 16 and 128 x 32 while more than 96 rows
+//    remain in large launches).
 //  * k_bf3w<TR,TC> — three-term split-BF16 (x = hi + mid + lo), six
 //    products of order <= 2 with v_mfma_f32_16x16x32_bf16, FP32 per task,
 //    FP64 across; four waves per TR x TC tile (64 x 64: C blocks more than
@@ -228,11 +230,14 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 // after task in task order and k ascending, so every C element sees the
 // same chain as in the one-wave kernels of the same precision.
 // ---------------------------------------------------------------------------
-template <int F, int TS>
+template <int F, int TR, int TC>
 __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 {
-  constexpr int T = TS / 32;   // 16 x 16 MFMA tiles per wave and direction
-  constexpr int E = TS / 16;   // elements of A and of B per loader lane and chunk
+  // waves: WR along the rows x WC along the columns of the tile
+  constexpr int WC = TC >= 32 ? 2 : 1, WR = 4 / WC;
+  constexpr int TRt = TR / WR / 16, TCt = TC / WC / 16;  // 16 x 16 MFMA tiles per wave
+  constexpr int EA = TR / 16, EB = TC / 16;  // elements of A / of B per loader lane and chunk
+  static_assert(TRt >= 1 && TCt >= 1 && EA % 2 == 0 && (EB == 1 || EB % 2 == 0), "tile shape");
   typedef typename std::conditional<F == 0, double, float>::type TL;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
@@ -240,19 +245,19 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int wr = (TS / 2) * (w & 1), wc = (TS / 2) * (w >> 1);
+  const int wr = (TR / WR) * (w % WR), wc = (TC / WC) * (w / WR);
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ TL sA[2][16][TS];  // [stage][k][row of the tile]
-  __shared__ TL sB[2][16][TS];  // [stage][k][column of the tile]
+  __shared__ TL sA[2][16][TR];  // [stage][k][row of the tile]
+  __shared__ TL sB[2][16][TC];  // [stage][k][column of the tile]
   // column of accumulator element v (the f64 and f32 MFMA results differ)
   auto colof = [&](int ct, int v) { return tile.n0 + wc + 16 * ct + (F == 0 ? 4 * v + lk : 4 * lk + v); };
-  v4d acc[T][T];
+  v4d acc[TRt][TCt];
 #pragma unroll
-  for (int rt = 0; rt < T; ++rt)
+  for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < T; ++ct)
+    for (int ct = 0; ct < TCt; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -260,32 +265,32 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero && cb.t1 > cb.t0) {
-    // loader lanes: A column kA (0..15) rows rA..rA+E-1; B column jB
-    // (0..TS-1) contracted rows kB..kB+E-1
-    const int kA = tid >> 4, rA = E * (tid & 15);
-    const int jB = tid / (256 / TS), kB = E * (tid % (256 / TS));
+    // loader lanes: A column kA (0..15) rows rA..rA+EA-1; B column jB
+    // (0..TC-1) contracted rows kB..kB+EB-1
+    const int kA = tid >> 4, rA = EA * (tid & 15);
+    const int jB = tid / (16 / EB), kB = EB * (tid % (16 / EB));
     const int ia = min(tile.m0 + rA, M - 1);
     const int jb = min(tile.n0 + jB, N - 1);
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
     TaskRec tr = p.tasks[tl];
-    double ra[E], rb[E];
+    double ra[EA], rb[EB];
     bool ka;       // this lane's A column is inside the task
     int kbb, Kb;   // contracted index of rb[0] and the task's K
     bool lastl;    // the loaded chunk is the last of its task (uniform)
     const double alpha = p.alpha;
     const bool a1 = p.alpha_one != 0;
-    v4f part[T][T];
+    v4f part[TRt][TCt];
 #pragma unroll
-    for (int rt = 0; rt < T; ++rt)
+    for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < T; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < TCt; ++ct) part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
 #define SHT_W_LOAD()                                                                       \
   {                                                                                        \
     const int K = tr.k;                                                                    \
     const int kk = kl + kA;                                                                \
     ka = kk < K;                                                                           \
     const double* pa = A + tr.a_off + static_cast<int64_t>(min(kk, K - 1)) * M + ia;       \
-    _Pragma("unroll") for (int e = 0; e < E; e += 2) {                                     \
+    _Pragma("unroll") for (int e = 0; e < EA; e += 2) {                                    \
       const v2d8 x = *reinterpret_cast<const v2d8*>(pa + e);                               \
       ra[e] = x.x;                                                                         \
       ra[e + 1] = x.y;                                                                     \
@@ -293,10 +298,13 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
     kbb = kl + kB;                                                                         \
     Kb = K;                                                                                \
     const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(kbb, K - 1);      \
-    _Pragma("unroll") for (int e = 0; e < E; e += 2) {                                     \
-      const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                               \
-      rb[e] = y.x;                                                                         \
-      rb[e + 1] = y.y;                                                                     \
+    if constexpr (EB == 1) rb[0] = pb[0];                                                  \
+    else {                                                                                 \
+      _Pragma("unroll") for (int e = 0; e < EB; e += 2) {                                  \
+        const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                             \
+        rb[e] = y.x;                                                                       \
+        rb[e + 1] = y.y;                                                                   \
+      }                                                                                    \
     }                                                                                      \
     lastl = kl + 16 >= K;                                                                  \
     if (tl < cb.t1) {                                                                      \
@@ -310,16 +318,15 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   }
 #define SHT_W_STORE(stg)                                                                   \
   {                                                                                        \
-    _Pragma("unroll") for (int e = 0; e < E; ++e) {                                        \
+    _Pragma("unroll") for (int e = 0; e < EA; ++e) {                                       \
       const double x = ka ? ra[e] : 0.0;                                                   \
+      if constexpr (F == 0) sA[stg][kA][rA + e] = a1 ? x : x * alpha;                      \
+      else sA[stg][kA][rA + e] = static_cast<float>(x);                                    \
+    }                                                                                      \
+    _Pragma("unroll") for (int e = 0; e < EB; ++e) {                                       \
       const double y = kbb + e < Kb ? rb[e] : 0.0;                                         \
-      if constexpr (F == 0) {                                                              \
-        sA[stg][kA][rA + e] = a1 ? x : x * alpha;                                          \
-        sB[stg][kB + e][jB] = y;                                                           \
-      } else {                                                                             \
-        sA[stg][kA][rA + e] = static_cast<float>(x);                                       \
-        sB[stg][kB + e][jB] = static_cast<float>(y);                                       \
-      }                                                                                    \
+      if constexpr (F == 0) sB[stg][kB + e][jB] = y;                                       \
+      else sB[stg][kB + e][jB] = static_cast<float>(y);                                    \
     }                                                                                      \
   }
     SHT_W_LOAD()
@@ -332,15 +339,15 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
       SHT_W_LOAD()                   // (past the end: a clamped re-read, unused)
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        TL a[T], b[T];
+        TL a[TRt], b[TCt];
 #pragma unroll
-        for (int rt = 0; rt < T; ++rt) a[rt] = sA[stg][4 * s + lk][wr + 16 * rt + li];
+        for (int rt = 0; rt < TRt; ++rt) a[rt] = sA[stg][4 * s + lk][wr + 16 * rt + li];
 #pragma unroll
-        for (int ct = 0; ct < T; ++ct) b[ct] = sB[stg][4 * s + lk][wc + 16 * ct + li];
+        for (int ct = 0; ct < TCt; ++ct) b[ct] = sB[stg][4 * s + lk][wc + 16 * ct + li];
 #pragma unroll
-        for (int rt = 0; rt < T; ++rt)
+        for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < T; ++ct) {
+          for (int ct = 0; ct < TCt; ++ct) {
             if constexpr (F == 0)
               acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
             else
@@ -350,9 +357,9 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
       if constexpr (F == 1) {
         if (lastc) {  // the task's block product is complete: add it in FP64
 #pragma unroll
-          for (int rt = 0; rt < T; ++rt)
+          for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-            for (int ct = 0; ct < T; ++ct) {
+            for (int ct = 0; ct < TCt; ++ct) {
 #pragma unroll
               for (int v = 0; v < 4; ++v) acc[rt][ct][v] += alpha * static_cast<double>(part[rt][ct][v]);
               part[rt][ct] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -369,9 +376,9 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 #undef SHT_W_STORE
   }
 #pragma unroll
-  for (int rt = 0; rt < T; ++rt)
+  for (int rt = 0; rt < TRt; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < T; ++ct)
+    for (int ct = 0; ct < TCt; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = tile.m0 + wr + 16 * rt + li;
@@ -728,7 +735,9 @@ int kclass_rows(KClass k)
     case KClass::Mfma64S:
     case KClass::F32S: return 32;
     case KClass::Mfma16q:
-    case KClass::Mfma32q: return 128;
+    case KClass::Mfma32q:
+    case KClass::F32Nq:
+    case KClass::F32Mq: return 128;
     default: return 64;
   }
 }
@@ -740,11 +749,13 @@ int kclass_cols(KClass k)
     case KClass::Exact16: return 16;
     case KClass::Exact32: return 32;
     case KClass::Mfma16:
-    case KClass::F32x16:
     case KClass::Mfma16h:
     case KClass::F32x16h:
     case KClass::Mfma16q:
-    case KClass::Bf16N: return 16;
+    case KClass::Bf16N:
+    case KClass::Mfma64N:
+    case KClass::F32N:
+    case KClass::F32Nq: return 16;
     case KClass::Mfma64W:
     case KClass::F32W:
     case KClass::Bf16W: return 64;
@@ -765,16 +776,20 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Exact32: hipLaunchKernelGGL(k_exact<32>, g, b, 0, s, a); break;
     case KClass::Mfma16: hipLaunchKernelGGL((k_mfma64<1, 4>), g, b, 0, s, a); break;
     case KClass::Mfma32: hipLaunchKernelGGL((k_mfma64<2, 4>), g, b, 0, s, a); break;
-    case KClass::F32x16: hipLaunchKernelGGL((k_mfma32<1, 4>), g, b, 0, s, a); break;
-    case KClass::F32x32: hipLaunchKernelGGL((k_mfma32<2, 4>), g, b, 0, s, a); break;
     case KClass::Mfma16h: hipLaunchKernelGGL((k_mfma64<1, 2>), g, b, 0, s, a); break;
     case KClass::F32x16h: hipLaunchKernelGGL((k_mfma32<1, 2>), g, b, 0, s, a); break;
     case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
-    case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64, 64>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64, 64>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64N: hipLaunchKernelGGL((k_mfmaw<0, 64, 16>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32N: hipLaunchKernelGGL((k_mfmaw<1, 64, 16>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64M: hipLaunchKernelGGL((k_mfmaw<0, 64, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32M: hipLaunchKernelGGL((k_mfmaw<1, 64, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32Nq: hipLaunchKernelGGL((k_mfmaw<1, 128, 16>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32Mq: hipLaunchKernelGGL((k_mfmaw<1, 128, 32>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32, 32>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::Bf16W: hipLaunchKernelGGL((k_bf3w<64, 64>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::Bf16S: hipLaunchKernelGGL((k_bf3w<32, 32>), g, dim3(4 * kWave), 0, s, a); break;
     case KClass::Bf16N: hipLaunchKernelGGL((k_bf3w<64, 16>), g, dim3(4 * kWave), 0, s, a); break;

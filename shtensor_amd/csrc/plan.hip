@@ -433,8 +433,10 @@ KClass half_rows(KClass k)
   switch (k) {
     case KClass::Mfma16: return KClass::Mfma16h;
     case KClass::Mfma32: return KClass::Mfma64S;
-    case KClass::F32x16: return KClass::F32x16h;
-    case KClass::F32x32: return KClass::F32S;
+    case KClass::Mfma64N: return KClass::Mfma16h;
+    case KClass::F32N: return KClass::F32x16h;
+    case KClass::Mfma64M: return KClass::Mfma64S;
+    case KClass::F32M: return KClass::F32S;
     default: return k;
   }
 }
@@ -445,6 +447,8 @@ KClass quad_rows(KClass k)
   switch (k) {
     case KClass::Mfma16: return KClass::Mfma16q;
     case KClass::Mfma32: return KClass::Mfma32q;
+    case KClass::F32N: return KClass::F32Nq;
+    case KClass::F32M: return KClass::F32Mq;
     default: return k;
   }
 }
@@ -508,24 +512,28 @@ void build_schedule(sht_plan* P)
   };
   std::vector<std::vector<T>> per(static_cast<int>(KClass::Count));
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
-  // 128-row FP64 tiles only for large launches (more than 64 64-row tiles
+  // 128-row FP64 / FP32 tiles only for large launches (more than 64 64-row tiles
   // per SIMD): there a wave's fixed cost (the dependent record loads before
   // its first MFMA) dominates, while small launches need every wave
   int64_t n64 = 0;
   for (int64_t c = 0; c < ncb; ++c) n64 += (cb[c].m + 63) / 64;
-  const bool quad = P->prec == SHT_PREC_FP64 && n64 > 64 * 1024;
+  const bool quad = (P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32) && n64 > 64 * 1024;
   for (int64_t c = 0; c < ncb; ++c) {
     const int M = cb[c].m, N = cb[c].n;
     KClass k;
     if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
-    else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32x16 : (N <= 32 ? KClass::F32x32 : KClass::F32W);
+    else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32N : (N <= 32 ? KClass::F32M : KClass::F32W);
     else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16N : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
-    else k = N <= 16 ? KClass::Mfma16 : (N <= 32 ? KClass::Mfma32 : KClass::Mfma64W);
+    else if (N > 32) k = KClass::Mfma64W;
+    // (FP64 large launches: one-wave 64-row tiles beside the 128-row ones,
+    // measured faster there than the four-wave 64-row forms)
+    else if (quad) k = N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
+    else k = N <= 16 ? KClass::Mfma64N : KClass::Mfma64M;
     const int cols = kclass_cols(k);
     const int64_t rk = row_key(P->C->indices[c]);
     for (int m0 = 0; m0 < M;) {
-      // row tiles: 128 rows while more than 96 remain (FP64 classes, large
-      // launches), then 64; the last row tile of an MFMA class takes the
+      // row tiles: 128 rows while more than 96 remain (FP64 and FP32 up to
+      // 32 columns, large launches), then 64; the last row tile of an MFMA class takes the
       // 32-row form when it can
       KClass kt = k;
       if (!exact && M - m0 <= 32) kt = half_rows(k);
@@ -1089,11 +1097,12 @@ const char* sht_kclass_name(int k)
   static_assert(sizeof(as_names) / sizeof(as_names[0]) == static_cast<size_t>(sht::AsClass::Count),
                 "a name per A-stationary class");
   if (k >= 100 && k < 100 + static_cast<int>(sizeof(as_names) / sizeof(as_names[0]))) return as_names[k - 100];
-  static const char* names[] = {"exact_n8",       "exact_n16",      "exact_n32",      "mfma64_n16",
-                                "mfma64_n32",     "mfma32_n16",     "mfma32_n32",     "mfma64_n16_r32",
-                                "mfma32_n16_r32", "mfma64_n64w",    "mfma32_n64w",    "bf16x3_n64w",
-                                "mfma64_n32s",    "mfma32_n32s",    "bf16x3_n32s",    "mfma64_n16_r128",
-                                "mfma64_n32_r128", "bf16x3_n16w"};
+  static const char* names[] = {"exact_n8",         "exact_n16",        "exact_n32",        "mfma64_n16",
+                                "mfma64_n32",       "mfma64_n16_r32",   "mfma32_n16_r32",   "mfma64_n64w",
+                                "mfma32_n64w",      "bf16x3_n64w",      "mfma64_n32s",      "mfma32_n32s",
+                                "bf16x3_n32s",      "mfma64_n16_r128",  "mfma64_n32_r128",  "mfma64_n16w",
+                                "mfma32_n16w",      "bf16x3_n16w",      "mfma64_n32w",      "mfma32_n32w",
+                                "mfma32_n16w_r128", "mfma32_n32w_r128"};
   static_assert(sizeof(names) / sizeof(names[0]) == static_cast<size_t>(sht::KClass::Count),
                 "a name per C-stationary class");
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";

@@ -48,17 +48,20 @@ struct ExecArgs {
 // kernel classes of the C-stationary launches
 enum class KClass : int {
   Exact8 = 0, Exact16, Exact32,      // fma chain, 128 rows x {8,16,32} cols
-  Mfma16, Mfma32,                    // FP64 MFMA, one wave, 64 rows x {16,32} cols
-  F32x16, F32x32,                    // FP32 MFMA (FP64 cross-task), one wave, 64 rows x {16,32} cols
-  Mfma16h, F32x16h,                  // the 16-column one-wave forms on 32-row tiles (a
-                                     // block's last row tile when at most 32 rows remain)
+  Mfma16, Mfma32,                    // FP64 MFMA, one wave, 64 rows x {16,32} cols (large
+                                     // launches, beside the 128-row forms)
+  Mfma16h, F32x16h,                  // one wave per 32 x 16 tile (a block's last row tile
+                                     // when at most 32 rows remain, at most 16 columns)
   Mfma64W, F32W, Bf16W,              // four waves per 64 x 64 tile (more than 32 columns)
   Mfma64S, F32S, Bf16S,              // four waves per 32 x 32 tile (17-32 columns: FP64 / FP32
                                      // for a block's last row tile of at most 32 rows,
                                      // split-BF16 for every row tile)
   Mfma16q, Mfma32q,                  // FP64 one-wave 128-row tiles (more than 96 rows
                                      // remain, large launches)
-  Bf16N,                             // split-BF16, four waves per 64 x 16 tile (at most 16 columns)
+  Mfma64N, F32N, Bf16N,              // four waves per 64 x 16 tile (at most 16 columns)
+  Mfma64M, F32M,                     // FP64 / FP32, four waves per 64 x 32 tile (17-32 columns)
+  F32Nq, F32Mq,                      // FP32, four waves per 128 x 16 / 128 x 32 tile (more
+                                     // than 96 rows remain, large launches)
   Count
 };
 
