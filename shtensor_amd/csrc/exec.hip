@@ -766,6 +766,22 @@ int kclass_cols(KClass k)
   }
 }
 
+int kclass_waves(KClass k)
+{
+  switch (k) {
+    case KClass::Exact8:
+    case KClass::Exact16:
+    case KClass::Exact32:
+    case KClass::Mfma16:
+    case KClass::Mfma32:
+    case KClass::Mfma16h:
+    case KClass::F32x16h:
+    case KClass::Mfma16q:
+    case KClass::Mfma32q: return 1;
+    default: return 4;
+  }
+}
+
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
 {
   if (a.ntiles <= 0) return;

@@ -493,22 +493,32 @@ void build_schedule(sht_plan* P)
   // blocks) go to one XCD, consecutively, and those blocks are shared in that
   // L2.  Rows are dealt to the XCD with the least accumulated cost (most
   // expensive first), the eight queues are interleaved, and short queues are
-  // padded with no-work tiles.  Otherwise (A and B alike, where the C block
-  // order already keeps tiles that share B blocks together) tiles are ordered
-  // by descending cost.
+  // padded with no-work tiles.  Otherwise (A and B alike) the unit dealt is
+  // a square of kSq x kSq C blocks (block rows and block columns), so the
+  // tiles an XCD runs at one time read a few block rows of A and a few block
+  // columns of B -- in launches of at least 32 waves per SIMD; shorter ones
+  // (a few rounds of waves, where the last round sets the time) order their
+  // tiles by descending cost instead.
   struct T {
     TileRec r;
     double cost;
-    int64_t row;  // rolled block index of C's row (free-A) axes
+    int64_t row;  // group key: C's block row (A the larger operand), else its square
   };
   const bool by_row = P->A->nelem >= 2 * P->B->nelem;
-  auto row_key = [&](int64_t id) {
+  auto axes_key = [&](int64_t id, const std::vector<int>& axes) {
     int64_t k = 0, mul = 1;
-    for (int ax : m.c_row) {
+    for (int ax : axes) {
       k += ((id / P->C->bstride[ax]) % P->C->nblk[ax]) * mul;
       mul *= P->C->nblk[ax];
     }
     return k;
+  };
+  constexpr int64_t kSq = 16;
+  int64_t ncolkeys = 1;
+  for (int ax : m.c_col) ncolkeys *= P->C->nblk[ax];
+  auto row_key = [&](int64_t id) {
+    if (by_row) return axes_key(id, m.c_row);
+    return (axes_key(id, m.c_row) / kSq) * (ncolkeys / kSq + 1) + axes_key(id, m.c_col) / kSq;
   };
   std::vector<std::vector<T>> per(static_cast<int>(KClass::Count));
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
@@ -552,13 +562,14 @@ void build_schedule(sht_plan* P)
   for (int k = 0; k < static_cast<int>(KClass::Count); ++k) {
     auto& v = per[k];
     if (v.empty()) continue;
-    if (!by_row) {
+    if (!by_row && static_cast<int64_t>(v.size()) * kclass_waves(static_cast<KClass>(k)) < 32 * 1024) {
       std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
       P->launches.push_back({static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size())});
       for (const T& t : v) all.push_back(t.r);
       continue;
     }
-    // rows: tiles grouped by row key, rows by descending total cost
+    // groups (rows or squares): tiles grouped by key, groups by descending
+    // total cost
     std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.row < y.row; });
     struct RowSpan {
       size_t b, e;

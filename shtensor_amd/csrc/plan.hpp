@@ -67,6 +67,7 @@ enum class KClass : int {
 
 int kclass_rows(KClass k);
 int kclass_cols(KClass k);
+int kclass_waves(KClass k);  // waves per workgroup (one tile per workgroup)
 
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
 
