@@ -785,7 +785,8 @@ int kclass_waves(KClass k)
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
 {
   if (a.ntiles <= 0) return;
-  const dim3 g(static_cast<unsigned>(a.ntiles)), b(kWave);
+  // one tile per workgroup of kclass_waves(k) waves
+  const dim3 g(static_cast<unsigned>(a.ntiles)), b(kclass_waves(k) * kWave);
   switch (k) {
     case KClass::Exact8: hipLaunchKernelGGL(k_exact<8>, g, b, 0, s, a); break;
     case KClass::Exact16: hipLaunchKernelGGL(k_exact<16>, g, b, 0, s, a); break;
@@ -796,19 +797,19 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::F32x16h: hipLaunchKernelGGL((k_mfma32<1, 2>), g, b, 0, s, a); break;
     case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
-    case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64, 64>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64, 64>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Mfma64N: hipLaunchKernelGGL((k_mfmaw<0, 64, 16>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32N: hipLaunchKernelGGL((k_mfmaw<1, 64, 16>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Mfma64M: hipLaunchKernelGGL((k_mfmaw<0, 64, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32M: hipLaunchKernelGGL((k_mfmaw<1, 64, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32Nq: hipLaunchKernelGGL((k_mfmaw<1, 128, 16>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32Mq: hipLaunchKernelGGL((k_mfmaw<1, 128, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Bf16W: hipLaunchKernelGGL((k_bf3w<64, 64>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Bf16S: hipLaunchKernelGGL((k_bf3w<32, 32>), g, dim3(4 * kWave), 0, s, a); break;
-    case KClass::Bf16N: hipLaunchKernelGGL((k_bf3w<64, 16>), g, dim3(4 * kWave), 0, s, a); break;
+    case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64, 64>), g, b, 0, s, a); break;
+    case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64, 64>), g, b, 0, s, a); break;
+    case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32, 32>), g, b, 0, s, a); break;
+    case KClass::Mfma64N: hipLaunchKernelGGL((k_mfmaw<0, 64, 16>), g, b, 0, s, a); break;
+    case KClass::F32N: hipLaunchKernelGGL((k_mfmaw<1, 64, 16>), g, b, 0, s, a); break;
+    case KClass::Mfma64M: hipLaunchKernelGGL((k_mfmaw<0, 64, 32>), g, b, 0, s, a); break;
+    case KClass::F32M: hipLaunchKernelGGL((k_mfmaw<1, 64, 32>), g, b, 0, s, a); break;
+    case KClass::F32Nq: hipLaunchKernelGGL((k_mfmaw<1, 128, 16>), g, b, 0, s, a); break;
+    case KClass::F32Mq: hipLaunchKernelGGL((k_mfmaw<1, 128, 32>), g, b, 0, s, a); break;
+    case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32, 32>), g, b, 0, s, a); break;
+    case KClass::Bf16W: hipLaunchKernelGGL((k_bf3w<64, 64>), g, b, 0, s, a); break;
+    case KClass::Bf16S: hipLaunchKernelGGL((k_bf3w<32, 32>), g, b, 0, s, a); break;
+    case KClass::Bf16N: hipLaunchKernelGGL((k_bf3w<64, 16>), g, b, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
   SHT_HIP(hipGetLastError());
