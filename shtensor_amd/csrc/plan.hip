@@ -497,7 +497,7 @@ void build_schedule(sht_plan* P)
   // a square of kSq x kSq C blocks (block rows and block columns), so the
   // tiles an XCD runs at one time read a few block rows of A and a few block
   // columns of B -- in launches of at least 32 waves per SIMD; shorter ones
-  // (a few rounds of waves, where the last round sets the time) order their
+  // (a few waves per SIMD, where the last wave to finish sets the time) order their
   // tiles by descending cost instead.
   struct T {
     TileRec r;
