@@ -340,6 +340,36 @@ def test_large_launch_128_row_tiles(ctx, prec, names):
     dC.close()
 
 
+@pytest.mark.parametrize("prec,name", [("fp64", "mfma64_n32s"), ("fp32", "mfma32_n32s"), ("bf16x3", "bf16x3_n32s")])
+def test_alike_operands_square_order(ctx, prec, name):
+    """A and B of the same size, and more than 32 four-wave tiles per SIMD:
+    the tiles are dealt to the XCD queues in squares of 16 x 16 C blocks
+    (9 216 C blocks of 32 x 32 in 96 x 96 block rows and columns, so the
+    squares at the edges are partial and the eight queues are padded); FP64
+    bit for bit equal to the oracle, FP32 and split-BF16 within their
+    tolerance."""
+    rng = np.random.default_rng(83)
+    expr = "ik,kj->ij"
+    bs = [32] * 96
+    A = random_bst(rng, [bs, bs], 0.25)
+    B = random_bst(rng, [bs, bs], 0.25)
+    Cp = orc.BlockSparse([bs, bs])
+    _, _, _, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
+    launches = [k for k, _ in plan.execute_timed(1.0, 0.0)]
+    assert name in launches, launches
+    sht.call("sht_device_synchronize")
+    plan.close()
+    assert np.array_equal(dC.get_indices(), R.indices)
+    if prec in EXACT:
+        assert np.array_equal(dC.download(), R.data)
+    else:
+        assert normwise(dC.download(), R.data) <= TOL[prec]
+    dC.close()
+
+
 @pytest.mark.parametrize("prec,scheds", [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP32, (0, 1, 2, 3)),
                                          (sht.PREC_BF16X3, (0, 1, 2))])
 def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
