@@ -183,8 +183,16 @@ class Kernel:
         self.ptr = p.value
 
     def call(self, a, b, c, nb_ops, stream=None):
-        """a, b, c: DeviceArray or raw device pointers."""
-        ptr = lambda x: C.c_void_p(x.ptr if isinstance(x, DeviceArray) else x)
+        """a, b, c: DeviceArray, raw device pointers, or contiguous numpy
+        arrays of the kernel's dtype (host memory: the library stages them
+        through device buffers and returns after C is written back)."""
+        def ptr(x):
+            if isinstance(x, DeviceArray):
+                return C.c_void_p(x.ptr)
+            if isinstance(x, np.ndarray):
+                assert x.dtype == self.dtype and x.flags["C_CONTIGUOUS"], "host operand dtype/layout"
+                return C.c_void_p(x.ctypes.data)
+            return C.c_void_p(x)
         call("sht_kernel_call", C.c_void_p(self.ptr), ptr(a), ptr(b), ptr(c), int(nb_ops), _sptr(stream))
 
     def get_info(self):

@@ -90,6 +90,17 @@ __global__ __launch_bounds__(256) void k_zero(uint4* p, size_t n16, unsigned cha
     for (size_t i = threadIdx.x; i < ntail; i += blockDim.x) tail[i] = 0;
 }
 
+// Block gather (BaseTensor::compress): record r copies n[r] 4-byte words
+// from word offset src[r] to word offset dst[r]; one workgroup per record.
+__global__ __launch_bounds__(256) void k_gather_blocks(const uint32_t* in, uint32_t* out, const int64_t* recs,
+                                                       int64_t nrec)
+{
+  for (int64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
+    const int64_t src = recs[3 * r], dst = recs[3 * r + 1], n = recs[3 * r + 2];
+    for (int64_t x = threadIdx.x; x < n; x += blockDim.x) out[dst + x] = in[src + x];
+  }
+}
+
 int grid_for(int64_t items, int64_t cap = 65536)
 {
   return static_cast<int>(items < 1 ? 1 : (items < cap ? items : cap));
@@ -117,6 +128,14 @@ void launch_block_norms(const TensorDesc& d, const void* data, bool f64, int nor
   else
     hipLaunchKernelGGL(k_block_norms<float>, dim3(g), dim3(256), 0, s, d, static_cast<const float*>(data), norm,
                        out);
+  SHT_HIP(hipGetLastError());
+}
+
+void launch_gather_blocks(const void* in, void* out, const int64_t* d_recs, int64_t nrec, hipStream_t s)
+{
+  if (nrec <= 0) return;
+  hipLaunchKernelGGL(k_gather_blocks, dim3(grid_for(nrec)), dim3(256), 0, s, static_cast<const uint32_t*>(in),
+                     static_cast<uint32_t*>(out), d_recs, nrec);
   SHT_HIP(hipGetLastError());
 }
 

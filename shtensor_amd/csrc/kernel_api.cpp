@@ -35,6 +35,12 @@ struct sht_kernel {
   bool alpha_zero = false, beta_zero = false;
   int* d_offs = nullptr;
   hipFunction_t jit_fn = nullptr;  // owned by the process-wide cache
+  bool jit_mfma = false;           // the specialised kernel is the MFMA form
+  int64_t jit_wpo = 1;             // its workgroups per op
+  // device staging for host-memory operands (grown on demand, freed with the kernel)
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  std::mutex stage_mu;
 };
 
 namespace sht {
@@ -82,46 +88,176 @@ struct JitEntry {
 std::mutex g_jit_mu;
 std::unordered_map<std::string, JitEntry> g_jit;
 
+// Launch shape of a specialised kernel.  FMA form: one lane per C element
+// of one op (grid-stride).  MFMA form (FP64, at least 16 rows and 4
+// contracted indices): one wave per group of R 16-row tiles x one 16-column
+// tile of one op, four waves (four row groups of the same op and column tile)
+// per workgroup.
+struct JitShape {
+  bool mfma = false;
+  int R = 1;      // 16-row tiles per wave
+  int NG = 1;     // row groups per op
+  int NU = 1;     // 16-column tiles per op
+  int WPO = 1;    // workgroups per op
+  bool bl = false;  // the column tile's B panel staged in LDS
+};
+
+constexpr int kKcWaves = 4;  // waves per workgroup of the MFMA form
+
+// Device staging of host operands: A, B, C of nb_ops ops, 256-byte aligned.
+struct StageLayout {
+  size_t a_bytes = 0, b_bytes = 0, c_bytes = 0;
+  size_t b_off = 0, c_off = 0, total = 0;
+};
+
+JitShape jit_shape(const sht_kernel& k)
+{
+  JitShape js;
+  const char* e = std::getenv("SHT_KCALL_FMA");
+  const bool force_fma = e && std::strcmp(e, "1") == 0;
+  if (k.dtype != SHT_FLOAT64 || k.M < 16 || k.K < 4 || force_fma) return js;
+  js.mfma = true;
+  js.R = std::min(4, (k.M + 15) / 16);
+  js.NG = (k.M + 16 * js.R - 1) / (16 * js.R);
+  js.NU = (k.N + 15) / 16;
+  js.WPO = (js.NG + kKcWaves - 1) / kKcWaves * js.NU;
+  js.bl = static_cast<int64_t>(k.K) * 16 * 8 <= 32768;
+  return js;
+}
+
 std::string jit_source(const sht_kernel& k, bool ca, bool cb, bool cc)
 {
+  const JitShape js = jit_shape(k);
   const char* T = k.dtype == SHT_FLOAT64 ? "double" : "float";
-  const char* FMA = k.dtype == SHT_FLOAT64 ? "__fma_rn" : "__fmaf_rn";
   std::ostringstream s;
   s << "typedef " << T << " T;\n"
     << "#define M " << k.M << "\n#define N " << k.N << "\n#define K " << k.K << "\n"
     << "#define AZ " << (k.alpha_zero ? 1 : 0) << "\n#define BZ " << (k.beta_zero ? 1 : 0) << "\n"
-    << "#define CA " << (ca ? 1 : 0) << "\n#define CB " << (cb ? 1 : 0) << "\n#define CC " << (cc ? 1 : 0) << "\n"
-    << "extern \"C\" __global__ __launch_bounds__(256) void sht_blockprod(const T* __restrict__ A,\n"
-       "    const T* __restrict__ B, T* C, long long nb_ops, const int* __restrict__ offs, T alpha, T beta)\n"
-       "{\n"
-       "  const int* ar = offs; const int* ac = ar + M; const int* br = ac + K;\n"
-       "  const int* bc = br + K; const int* cr = bc + N; const int* cc = cr + M;\n"
-       "  const long long per = (long long)M * N;\n"
-       "  const long long total = nb_ops * per;\n"
-       "  const long long stride = (long long)gridDim.x * blockDim.x;\n"
-       "  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {\n"
-       "    const long long op = g / per;\n"
-       "    const int r = (int)(g - op * per);\n"
-       "    const int j = r / M, i = r - j * M;\n"
-       "    const T* a = A + op * (long long)(M * K);\n"
-       "    const T* b = B + op * (long long)(K * N);\n"
-       "    T* c = C + op * per;\n"
-       "    const int oc = CC ? (i + j * M) : (cr[i] + cc[j]);\n"
-       "    T acc = BZ ? T(0) : c[oc] * beta;\n"
-       "    if (!AZ) {\n"
-       "      const int ai = CA ? i : ar[i];\n"
-       "      const int bj = CB ? j * K : bc[j];\n"
-       "#pragma unroll 8\n"
-       "      for (int k = 0; k < K; ++k) {\n"
-       "        const T av = a[ai + (CA ? k * M : ac[k])] * alpha;\n"
-       "        acc = "
-    << FMA
-    << "(av, b[(CB ? k : br[k]) + bj], acc);\n"
-       "      }\n"
-       "    }\n"
-       "    c[oc] = acc;\n"
-       "  }\n"
-       "}\n";
+    << "#define CA " << (ca ? 1 : 0) << "\n#define CB " << (cb ? 1 : 0) << "\n#define CC " << (cc ? 1 : 0) << "\n";
+  if (!js.mfma) {
+    const char* FMA = k.dtype == SHT_FLOAT64 ? "__fma_rn" : "__fmaf_rn";
+    s << "extern \"C\" __global__ __launch_bounds__(256) void sht_blockprod(const T* __restrict__ A,\n"
+         "    const T* __restrict__ B, T* C, long long nb_ops, const int* __restrict__ offs, T alpha, T beta)\n"
+         "{\n"
+         "  const int* ar = offs; const int* ac = ar + M; const int* br = ac + K;\n"
+         "  const int* bc = br + K; const int* cr = bc + N; const int* cc = cr + M;\n"
+         "  const long long per = (long long)M * N;\n"
+         "  const long long total = nb_ops * per;\n"
+         "  const long long stride = (long long)gridDim.x * blockDim.x;\n"
+         "  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {\n"
+         "    const long long op = g / per;\n"
+         "    const int r = (int)(g - op * per);\n"
+         "    const int j = r / M, i = r - j * M;\n"
+         "    const T* a = A + op * (long long)(M * K);\n"
+         "    const T* b = B + op * (long long)(K * N);\n"
+         "    T* c = C + op * per;\n"
+         "    const int oc = CC ? (i + j * M) : (cr[i] + cc[j]);\n"
+         "    T acc = BZ ? T(0) : c[oc] * beta;\n"
+         "    if (!AZ) {\n"
+         "      const int ai = CA ? i : ar[i];\n"
+         "      const int bj = CB ? j * K : bc[j];\n"
+         "#pragma unroll 8\n"
+         "      for (int k = 0; k < K; ++k) {\n"
+         "        const T av = a[ai + (CA ? k * M : ac[k])] * alpha;\n"
+         "        acc = "
+      << FMA
+      << "(av, b[(CB ? k : br[k]) + bj], acc);\n"
+         "      }\n"
+         "    }\n"
+         "    c[oc] = acc;\n"
+         "  }\n"
+         "}\n";
+    return s.str();
+  }
+  // FP64 MFMA form.  v_mfma_f64_16x16x4f64 accumulates its four products as
+  // a sequential fused multiply-add chain in k order, so C(i,j) =
+  // fma(alpha*A(i,k), B(k,j), .) over k ascending -- the reference JIT's chain
+  // (src/Kernel/KernelImpl.cpp:594-710) -- when the tile is computed
+  // transposed (D = B^T x (alpha A)^T: products commute exactly): lane l then
+  // holds rows i0 + l%16 of columns j0 + 4v + l/16, so C loads and stores
+  // cover 16 consecutive rows per column.  The K % 4 tail runs as scalar fma
+  // (no zero padding: a padded product could turn a -0 chain into +0).  Rows
+  // past M and columns past N read clamped (valid) addresses and are not
+  // stored.
+  s << "#define R " << js.R << "\n#define NG " << js.NG << "\n#define NU " << js.NU << "\n#define WPO " << js.WPO
+    << "\n#define BL " << (js.bl ? 1 : 0) << "\n#define NW " << kKcWaves << "\n"
+    << R"SRC(typedef double v4d __attribute__((ext_vector_type(4)));
+#define K4 (K / 4 * 4)
+__device__ __forceinline__ int aoff(const int* ar, const int* ac, int i, int k) { return CA ? i + k * M : ar[i] + ac[k]; }
+__device__ __forceinline__ int boff(const int* br, const int* bc, int k, int j) { return CB ? k + j * K : br[k] + bc[j]; }
+__device__ __forceinline__ int coff(const int* cr, const int* cc, int i, int j) { return CC ? i + j * M : cr[i] + cc[j]; }
+extern "C" __global__ __launch_bounds__(64 * NW) void sht_blockprod(const double* __restrict__ A,
+    const double* __restrict__ B, double* C, long long nb_ops, const int* __restrict__ offs, double alpha, double beta)
+{
+  const int* ar = offs; const int* ac = ar + M; const int* br = ac + K;
+  const int* bc = br + K; const int* cr = bc + N; const int* cc = cr + M;
+#if BL
+  __shared__ double bs[K * 16];
+#endif
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const long long nwg = nb_ops * WPO;
+  for (long long wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
+    const long long op = wg / WPO;
+    const int rem = (int)(wg - op * WPO);
+    const int u = rem % NU, g = (rem / NU) * NW + wave;
+    const double* a = A + op * (long long)(M * K);
+    const double* b = B + op * (long long)(K * N);
+    double* c = C + op * (long long)(M * N);
+    const int j0 = 16 * u;
+#if BL
+    __syncthreads();  // the previous item's reads of bs are done
+    for (int x = threadIdx.x; x < K * 16; x += 64 * NW) {
+      const int kk = x % K, jj = x / K;
+      bs[kk * 16 + jj] = b[boff(br, bc, kk, min(j0 + jj, N - 1))];
+    }
+    __syncthreads();
+#endif
+    if (g >= NG) continue;
+    const int i0 = 16 * R * g;
+    int ri[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) ri[t] = min(i0 + 16 * t + li, M - 1);
+    int cj[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) cj[v] = min(j0 + 4 * v + lk, N - 1);
+    v4d acc[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[t][v] = BZ ? 0.0 : c[coff(cr, cc, ri[t], cj[v])] * beta;
+    if (!AZ) {
+      const int jb = min(j0 + li, N - 1);
+#pragma unroll 4
+      for (int k0 = 0; k0 < K4; k0 += 4) {
+#if BL
+        const double bf = bs[(k0 + lk) * 16 + li];
+#else
+        const double bf = b[boff(br, bc, k0 + lk, jb)];
+#endif
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+          const double af = a[aoff(ar, ac, ri[t], k0 + lk)] * alpha;
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf, af, acc[t], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int k = K4; k < K; ++k)
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+          const double av = a[aoff(ar, ac, ri[t], k)] * alpha;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[t][v] = __fma_rn(av, b[boff(br, bc, k, cj[v])], acc[t][v]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (i0 + 16 * t + li < M && j0 + 4 * v + lk < N) c[coff(cr, cc, ri[t], cj[v])] = acc[t][v];
+  }
+}
+)SRC";
   return s.str();
 }
 
@@ -225,6 +361,83 @@ hipFunction_t jit_blockprod(const sht_kernel& k, bool load)
   return e.fn;
 }
 
+// Enqueue the block products of nb_ops packed ops (device pointers) on s.
+void kernel_launch(const sht_kernel& k, const void* a, const void* b, void* c, int64_t nb_ops, hipStream_t s)
+{
+  if (k.jit_fn) {
+    long long nops = nb_ops;
+    unsigned g, bs;
+    if (k.jit_mfma) {
+      const int64_t nwg = nb_ops * k.jit_wpo;
+      g = static_cast<unsigned>(nwg < 1048576 ? nwg : 1048576);
+      bs = 64 * kKcWaves;
+    } else {
+      const int64_t total = nb_ops * static_cast<int64_t>(k.M) * k.N;
+      const int64_t blocks = (total + 255) / 256;
+      g = static_cast<unsigned>(blocks < 262144 ? blocks : 262144);
+      bs = 256;
+    }
+    if (k.dtype == SHT_FLOAT64) {
+      double al = k.alpha, be = k.beta;
+      void* args[] = {&a, &b, &c, &nops, const_cast<int**>(&k.d_offs), &al, &be};
+      SHT_HIP(hipModuleLaunchKernel(k.jit_fn, g, 1, 1, bs, 1, 1, 0, s, args, nullptr));
+    } else {
+      float al = static_cast<float>(k.alpha), be = static_cast<float>(k.beta);
+      void* args[] = {&a, &b, &c, &nops, const_cast<int**>(&k.d_offs), &al, &be};
+      SHT_HIP(hipModuleLaunchKernel(k.jit_fn, g, 1, 1, bs, 1, 1, 0, s, args, nullptr));
+    }
+    return;
+  }
+  BlockProdArgs p{a, b, c, nb_ops, k.M, k.N, k.K, k.d_offs, k.alpha, k.beta, k.alpha_zero ? 1 : 0,
+                  k.beta_zero ? 1 : 0};
+  launch_blockprod(p, k.dtype == SHT_FLOAT64, s);
+}
+
+// True for memory the device does not own (pageable or pinned host memory).
+bool is_host_ptr(const void* p)
+{
+  hipPointerAttribute_t at{};
+  const hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory
+    return true;
+  }
+  return at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged;
+}
+
+StageLayout stage_layout(const sht_kernel& k, int64_t nb_ops)
+{
+  const size_t es = k.dtype == SHT_FLOAT64 ? 8 : 4;
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  StageLayout L;
+  L.a_bytes = static_cast<size_t>(nb_ops) * k.M * k.K * es;
+  L.b_bytes = static_cast<size_t>(nb_ops) * k.K * k.N * es;
+  L.c_bytes = static_cast<size_t>(nb_ops) * k.M * k.N * es;
+  L.b_off = up(L.a_bytes);
+  L.c_off = L.b_off + up(L.b_bytes);
+  L.total = L.c_off + up(L.c_bytes);
+  return L;
+}
+
+// One call with every host operand copied through `stage` (device memory of
+// at least L.total bytes); returns after C is back in host memory.
+void kernel_call_staged(const sht_kernel& k, const void* a, const void* b, void* c, int64_t nb_ops, hipStream_t s,
+                        void* stage, const StageLayout& L)
+{
+  char* base = static_cast<char*>(stage);
+  const bool ha = is_host_ptr(a), hb = is_host_ptr(b), hc = is_host_ptr(c);
+  const void* da = ha ? base : a;
+  const void* db = hb ? base + L.b_off : b;
+  void* dc = hc ? base + L.c_off : c;
+  if (ha) SHT_HIP(hipMemcpyAsync(base, a, L.a_bytes, hipMemcpyHostToDevice, s));
+  if (hb) SHT_HIP(hipMemcpyAsync(base + L.b_off, b, L.b_bytes, hipMemcpyHostToDevice, s));
+  // C is read only when beta != 0 (KernelImpl.cpp:478-479)
+  if (hc && !k.beta_zero) SHT_HIP(hipMemcpyAsync(base + L.c_off, c, L.c_bytes, hipMemcpyHostToDevice, s));
+  kernel_launch(k, da, db, dc, nb_ops, s);
+  if (hc) SHT_HIP(hipMemcpyAsync(c, base + L.c_off, L.c_bytes, hipMemcpyDeviceToHost, s));
+  SHT_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace sht
 
 using sht::Error;
@@ -295,6 +508,11 @@ sht_status sht_kernel_create(const char* expr, const int* sizes_a, int na, const
     SHT_HIP(hipMalloc(&k->d_offs, offs.size() * sizeof(int)));
     SHT_HIP(hipMemcpy(k->d_offs, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice));
     k->jit_fn = sht::jit_blockprod(*k, true);
+    if (k->jit_fn) {
+      const sht::JitShape js = sht::jit_shape(*k);
+      k->jit_mfma = js.mfma;
+      k->jit_wpo = js.WPO;
+    }
     *out = k.release();
   });
 }
@@ -306,25 +524,25 @@ sht_status sht_kernel_call(sht_kernel* k, const void* a, const void* b, void* c,
     if (nb_ops <= 0) return;  // the reference's do-while runs one op for 0; here 0 is a no-op
     SHT_REQUIRE(a && b && c, SHT_ERR_INVALID, "null data pointer");
     hipStream_t s = sht::as_stream(stream);
-    if (k->jit_fn) {
-      const int64_t total = nb_ops * static_cast<int64_t>(k->M) * k->N;
-      const int64_t blocks = (total + 255) / 256;
-      const unsigned g = static_cast<unsigned>(blocks < 262144 ? blocks : 262144);
-      long long nops = nb_ops;
-      if (k->dtype == SHT_FLOAT64) {
-        double al = k->alpha, be = k->beta;
-        void* args[] = {&a, &b, &c, &nops, &k->d_offs, &al, &be};
-        SHT_HIP(hipModuleLaunchKernel(k->jit_fn, g, 1, 1, 256, 1, 1, 0, s, args, nullptr));
-      } else {
-        float al = static_cast<float>(k->alpha), be = static_cast<float>(k->beta);
-        void* args[] = {&a, &b, &c, &nops, &k->d_offs, &al, &be};
-        SHT_HIP(hipModuleLaunchKernel(k->jit_fn, g, 1, 1, 256, 1, 1, 0, s, args, nullptr));
-      }
+    const bool ha = sht::is_host_ptr(a), hb = sht::is_host_ptr(b), hc = sht::is_host_ptr(c);
+    if (!ha && !hb && !hc) {
+      sht::kernel_launch(*k, a, b, c, nb_ops, s);
       return;
     }
-    sht::BlockProdArgs p{a, b, c, nb_ops, k->M, k->N, k->K, k->d_offs, k->alpha, k->beta,
-                         k->alpha_zero ? 1 : 0, k->beta_zero ? 1 : 0};
-    sht::launch_blockprod(p, k->dtype == SHT_FLOAT64, s);
+    // Host operands (the reference's callers pass std::vector data,
+    // test/test_kernel.cpp:143-150): staged through device buffers owned by
+    // the kernel; the call then completes before it returns, as the
+    // reference's does.
+    std::lock_guard<std::mutex> lk(k->stage_mu);
+    const sht::StageLayout L = sht::stage_layout(*k, nb_ops);
+    if (k->stage_bytes < L.total) {
+      if (k->d_stage) SHT_HIP(hipFree(k->d_stage));
+      k->d_stage = nullptr;
+      k->stage_bytes = 0;
+      SHT_HIP(hipMalloc(&k->d_stage, L.total));
+      k->stage_bytes = L.total;
+    }
+    sht::kernel_call_staged(*k, a, b, c, nb_ops, s, k->d_stage, L);
   });
 }
 
@@ -349,7 +567,9 @@ sht_status sht_kernel_info(const sht_kernel* k, char* buf, size_t len)
       out += "scatter row: " + sht::join(k->srow[t]) + "\n";
       out += "scatter col: " + sht::join(k->scol[t]) + "\n";
     }
-    out += std::string("  Backend: HIP gfx950, ") + (k->jit_fn ? "hipRTC-specialised" : "generic") + "\n";
+    out += std::string("  Backend: HIP gfx950, ") +
+           (k->jit_fn ? (k->jit_mfma ? "hipRTC-specialised (FP64 MFMA)" : "hipRTC-specialised (FMA)") : "generic") +
+           "\n";
     std::snprintf(buf, len, "%s", out.c_str());
   });
 }
@@ -359,6 +579,7 @@ sht_status sht_kernel_destroy(sht_kernel* k)
   return guard([&] {
     if (!k) return;
     if (k->d_offs) SHT_HIP(hipFree(k->d_offs));
+    if (k->d_stage) SHT_HIP(hipFree(k->d_stage));
     delete k;
   });
 }

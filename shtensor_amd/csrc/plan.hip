@@ -261,6 +261,12 @@ void check_tensors(const sht_tensor* A, const sht_tensor* B, const sht_tensor* C
   SHT_REQUIRE(A->dtype == SHT_FLOAT64 && B->dtype == SHT_FLOAT64 && C->dtype == SHT_FLOAT64, SHT_ERR_UNSUPPORTED,
               "contraction supports FLOAT64 tensors (precision forms select the block arithmetic)");
   SHT_REQUIRE(A != C && B != C, SHT_ERR_INVALID, "C must not alias A or B");
+  // Tensor::filter without compress leaves negated ids (-id-1) in the index
+  // list (Tensor.h:336-363); such a tensor has no valid block coordinates
+  // until BaseTensor::compress (BaseTensor.cpp:253-354) runs.
+  for (const sht_tensor* t : {A, B, C})
+    SHT_REQUIRE(!t->reserved || std::all_of(t->indices.begin(), t->indices.end(), [](int64_t id) { return id >= 0; }),
+                SHT_ERR_STATE, "tensor holds filtered blocks: compress the tensor first");
 }
 
 // Build the task list and C pattern on the device.

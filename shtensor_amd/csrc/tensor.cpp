@@ -477,7 +477,7 @@ sht_status sht_tensor_compress(sht_tensor* t)
       }
     }
     if (static_cast<int64_t>(keep_ids.size()) == nb) return;
-    // compact into a fresh allocation: block-by-block device copies
+    // compact into a fresh allocation: one gather launch over all kept blocks
     const std::vector<int64_t> old_off = t->offsets;
     std::vector<int64_t> old_ids = t->indices;
     for (auto& x : old_ids) if (x < 0) x = -x - 1;
@@ -489,18 +489,30 @@ sht_status sht_tensor_compress(sht_tensor* t)
     t->indices = std::move(keep_ids);
     const size_t es = t->elem_size();
     std::vector<int64_t> ids = t->indices;
+    int64_t* d_recs = nullptr;
     try {
       sht::tensor_reserve_sorted(t, std::move(ids));
+      const int64_t w = static_cast<int64_t>(es / 4);  // 4-byte words per element
+      std::vector<int64_t> recs;
+      recs.reserve(3 * keep_src.size());
       for (size_t q = 0; q < keep_src.size(); ++q) {
         const int64_t p = keep_src[q];
-        const int64_t vol = t->volume(old_ids[p]);
-        SHT_HIP(hipMemcpy(static_cast<char*>(t->d_data) + t->offsets[q] * es,
-                          static_cast<char*>(old_data) + old_off[p] * es, vol * es, hipMemcpyDeviceToDevice));
+        recs.push_back(old_off[p] * w);
+        recs.push_back(t->offsets[q] * w);
+        recs.push_back(t->volume(old_ids[p]) * w);
+      }
+      if (!recs.empty()) {
+        d_recs = static_cast<int64_t*>(c->arena.allocate(recs.size() * sizeof(int64_t)));
+        SHT_HIP(hipMemcpy(d_recs, recs.data(), recs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        sht::launch_gather_blocks(old_data, t->d_data, d_recs, static_cast<int64_t>(keep_src.size()), nullptr);
+        SHT_HIP(hipStreamSynchronize(nullptr));
       }
     } catch (...) {
+      if (d_recs) c->arena.release(d_recs);
       c->arena.release(old_data);
       throw;
     }
+    if (d_recs) c->arena.release(d_recs);
     c->arena.release(old_data);
   });
 }

@@ -2,6 +2,7 @@
 the oracle's XMM chain: bitwise for every expression of
 test/test_kernel.cpp, both dtypes, generic and hipRTC-specialised paths."""
 import os
+import zlib
 
 import numpy as np
 import pytest
@@ -45,7 +46,7 @@ def run_gpu(expr, sa, sb, sc, alpha, beta, a, b, c, nb, dtype):
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.75, 1.5)])
 def test_kernel_bitwise_vs_oracle(expr, sa, sb, sc, dtype, alpha, beta):
     print("kernel", expr, sa, sb, sc, np.dtype(dtype).name, alpha, beta, flush=True)
-    rng = np.random.default_rng(abs(hash((expr, tuple(sa)))) % 2 ** 32)
+    rng = np.random.default_rng(zlib.crc32(repr((expr, sa)).encode()))
     nb = 7
     a = rng.uniform(-1, 1, nb * int(np.prod(sa))).astype(dtype)
     b = rng.uniform(-1, 1, nb * int(np.prod(sb))).astype(dtype)
@@ -54,6 +55,76 @@ def test_kernel_bitwise_vs_oracle(expr, sa, sb, sc, dtype, alpha, beta):
     orc.kernel_xmm(expr, sa, sb, sc, alpha, beta, a, b, want, nb)
     got = run_gpu(expr, sa, sb, sc, alpha, beta, a, b, c, nb, dtype)
     assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+# FP64 MFMA form (M >= 16, K >= 4): partial row/column tiles, K % 4 tails,
+# K above the LDS-staged panel limit (256), permuted maps on every operand.
+MFMA_CASES = [
+    ("ik,kj->ij", [16, 4], [4, 16], [16, 16]),
+    ("ik,kj->ij", [33, 7], [7, 21], [33, 21]),
+    ("ik,kj->ij", [100, 300], [300, 5], [100, 5]),
+    ("ik,kj->ij", [17, 258], [258, 40], [17, 40]),
+    ("ijk,kl->ijl", [16, 16, 16], [16, 16], [16, 16, 16]),
+    ("ijkl,klm->ijm", [5, 7, 9, 11], [9, 11, 13], [5, 7, 13]),
+    ("kil,ljk->ij", [6, 20, 5], [5, 9, 6], [20, 9]),
+    ("ik,kjl->ilj", [40, 6], [6, 5, 3], [40, 3, 5]),
+    ("jik,kmj->im", [6, 24, 8], [8, 19, 6], [24, 19]),
+]
+
+
+@pytest.mark.parametrize("expr,sa,sb,sc", MFMA_CASES)
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.75, 1.5), (-1.0, -2.0)])
+def test_kernel_mfma_form_bitwise(expr, sa, sb, sc, alpha, beta):
+    print("kernel mfma", expr, sa, sb, sc, alpha, beta, flush=True)
+    rng = np.random.default_rng(zlib.crc32(repr((expr, sa, alpha)).encode()))
+    nb = 5
+    a = rng.uniform(-1, 1, nb * int(np.prod(sa)))
+    b = rng.uniform(-1, 1, nb * int(np.prod(sb)))
+    c = rng.uniform(-1, 1, nb * int(np.prod(sc)))
+    k = sht.Kernel(expr, sa, sb, sc, alpha, beta)
+    assert "FP64 MFMA" in k.get_info()
+    k.close()
+    want = c.copy()
+    orc.kernel_xmm(expr, sa, sb, sc, alpha, beta, a, b, want, nb)
+    got = run_gpu(expr, sa, sb, sc, alpha, beta, a, b, c, nb, np.float64)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+def test_kernel_mfma_signed_zero_chain():
+    # a chain that stays -0.0 (beta = -1 on C = +0, zero products of negative
+    # B entries): zero padding of the K % 4 tail would turn it into +0.0
+    sa, sb, sc = [16, 5], [5, 16], [16, 16]
+    a = np.zeros(80)
+    b = -np.ones(80)
+    c = np.zeros(256)
+    want = c.copy()
+    orc.kernel_xmm("ik,kj->ij", sa, sb, sc, 1.0, -1.0, a, b, want, 1)
+    assert np.signbit(want).all()
+    got = run_gpu("ik,kj->ij", sa, sb, sc, 1.0, -1.0, a, b, c, 1, np.float64)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_kernel_host_pointers(dtype):
+    # the reference's caller passes std::vector data (test/test_kernel.cpp:143-150)
+    expr, sa, sb, sc = "ijk,kl->ijl", [16, 16, 16], [16, 16], [16, 16, 16]
+    rng = np.random.default_rng(9)
+    nb = 11
+    a = rng.uniform(-1, 1, nb * 4096).astype(dtype)
+    b = rng.uniform(-1, 1, nb * 256).astype(dtype)
+    c = rng.uniform(-1, 1, nb * 4096).astype(dtype)
+    want = c.copy()
+    orc.kernel_xmm(expr, sa, sb, sc, 0.5, 1.0, a, b, want, nb)
+    k = sht.Kernel(expr, sa, sb, sc, 0.5, 1.0, sht.XMM, dtype)
+    k.call(a, b, c, nb)  # host memory in and out; returns when C is written
+    assert np.array_equal(c.view(np.uint8), want.view(np.uint8))
+    # mixed: device A, host B and C
+    c2 = want.copy()
+    orc.kernel_xmm(expr, sa, sb, sc, 0.5, 1.0, a, b, want, nb)
+    da = sht.DeviceArray(a)
+    k.call(da, b, c2, nb)
+    assert np.array_equal(c2.view(np.uint8), want.view(np.uint8))
+    k.close()
 
 
 @pytest.mark.parametrize("expr,sa,sb,sc", CASES[:5])

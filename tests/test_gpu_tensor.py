@@ -93,6 +93,53 @@ def test_filter_and_compress(ctx):
     assert (T.download() == 1.0).all()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_compress_keeps_block_values(ctx, dtype):
+    # BaseTensor::compress (BaseTensor.cpp:253-354), fixed: every surviving
+    # block keeps its own values at its re-packed offset (one gather launch)
+    rng = np.random.default_rng(11)
+    bs = [[3, 7, 2, 5], [4, 1, 6], [2, 9, 3]]
+    T = sht.Tensor(ctx, bs, dtype)
+    T.reserve_all()
+    info = T.get_sparse_info()
+    data = rng.uniform(-1, 1, T.nelem()).astype(dtype)
+    small = rng.random(len(info["indices"])) < 0.4
+    before = {}
+    for q, (bid, off) in enumerate(zip(info["indices"], info["offsets"])):
+        n = int(np.prod(T.block_dims(int(bid))))
+        if small[q]:
+            data[off:off + n] *= dtype(1e-9)
+        else:
+            before[int(bid)] = data[off:off + n].copy()
+    T.upload(data)
+    T.filter(1e-6, sht.FROBENIUS, compress=True)
+    info = T.get_sparse_info()
+    assert sorted(before) == list(info["indices"])
+    out = T.download()
+    for bid, off in zip(info["indices"], info["offsets"]):
+        b = before[int(bid)]
+        assert np.array_equal(out[off:off + len(b)], b)
+
+
+def test_filtered_tensor_rejected_by_plan(ctx):
+    # filter without compress leaves negated ids: a plan must refuse the
+    # tensor before any kernel reads its block coordinates
+    bs = [[4, 4], [4, 4]]
+    A = sht.Tensor(ctx, bs)
+    A.reserve_all()
+    A.upload(np.r_[np.zeros(16), np.ones(48)])
+    A.filter(1e-6, sht.FROBENIUS, compress=False)
+    assert (A.get_sparse_info()["indices"] < 0).any()
+    B = sht.Tensor(ctx, bs)
+    B.reserve_all()
+    C = sht.Tensor(ctx, bs)
+    with pytest.raises(sht.ShtError, match="compress the tensor first"):
+        sht.Plan(ctx, "ik,kj->ij", A, B, C)
+    A.compress()
+    P = sht.Plan(ctx, "ik,kj->ij", A, B, C)
+    P.close()
+
+
 def test_filter_small_values_fixed_norm(ctx):
     # the reference's int accumulator (BlockSpan.h:147) would drop |v| < 1 blocks
     T = sht.Tensor(ctx, [[2, 2], [2]])
