@@ -94,9 +94,22 @@ sht_status sht_kernel_create(const char* expr, const int* sizes_a, int na, const
                              const int* sizes_c, int nc, double alpha, double beta, sht_dtype dtype,
                              sht_kernel_type kernel_type, sht_kernel** out);
 /* Kernel<T>::call (Kernel.h:88-91): nb_ops packed same-shape products,
- * C_op = beta*C_op + sum_k (alpha*A_op) B_op; a/b/c are device pointers.
- * nb_ops <= 0 is a no-op.  Asynchronous on `stream`.                        */
+ * C_op = beta*C_op + sum_k (alpha*A_op) B_op.  nb_ops <= 0 is a no-op.
+ * Device pointers: asynchronous on `stream`.  Host pointers (pageable or
+ * pinned, the reference's std::vector data) are staged through device
+ * buffers the kernel allocates and keeps; the call then returns after C is
+ * written back.                                                             */
 sht_status sht_kernel_call(sht_kernel* k, const void* a, const void* b, void* c, int64_t nb_ops, void* stream);
+/* No counterpart in the reference: sht_kernel_call for callers that cannot
+ * allocate device memory (code under HIP graph capture is not meant: the
+ * call synchronizes; tight serving loops, harnesses that forbid hipMalloc).
+ * Host operands are staged through one fixed 64 MiB device array per GPU
+ * instead of buffers the kernel allocates; device operands are used in
+ * place.  Fails with SHT_ERR_INVALID when the staged operands of nb_ops ops
+ * (A, B and C, each rounded up to 256 bytes) exceed 64 MiB.  Returns after C
+ * is written.  Calls that overlap in time on one GPU share the array and
+ * race.  No hipMalloc or hipFree.                                           */
+sht_status sht_kernel_call_static(sht_kernel* k, const void* a, const void* b, void* c, int64_t nb_ops, void* stream);
 /* KernelImpl::get_info (KernelImpl.cpp:773-794) text into buf.             */
 sht_status sht_kernel_info(const sht_kernel* k, char* buf, size_t len);
 sht_status sht_kernel_destroy(sht_kernel* k);

@@ -438,6 +438,37 @@ void kernel_call_staged(const sht_kernel& k, const void* a, const void* b, void*
   SHT_HIP(hipStreamSynchronize(s));
 }
 
+// ---- allocation-free staging (sht_kernel_call_static) ---------------------
+// A fixed device array stages host operands for callers that cannot allocate
+// device memory.  Host code never passes the array's name to a kernel or a
+// copy: the name is a host-side stand-in; each device's own address comes
+// from hipGetSymbolAddress (looked up once per device, with that device
+// current) and only that address is used.  Calls on one device share the
+// array: calls that overlap in time on one device race.
+constexpr size_t kStaticStageBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticStageBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+void* static_stage_ptr()
+{
+  int dev = 0;
+  SHT_HIP(hipGetDevice(&dev));
+  SHT_REQUIRE(dev >= 0 && dev < kMaxDevices, SHT_ERR_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    SHT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SHT_REQUIRE(ptr != nullptr, SHT_ERR_HIP, "hipGetSymbolAddress returned no address for the staging array");
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
+}
+
 }  // namespace sht
 
 using sht::Error;
@@ -543,6 +574,23 @@ sht_status sht_kernel_call(sht_kernel* k, const void* a, const void* b, void* c,
       k->stage_bytes = L.total;
     }
     sht::kernel_call_staged(*k, a, b, c, nb_ops, s, k->d_stage, L);
+  });
+}
+
+sht_status sht_kernel_call_static(sht_kernel* k, const void* a, const void* b, void* c, int64_t nb_ops,
+                                  void* stream)
+{
+  return guard([&] {
+    SHT_REQUIRE(k, SHT_ERR_INVALID, "null kernel");
+    if (nb_ops <= 0) return;
+    SHT_REQUIRE(a && b && c, SHT_ERR_INVALID, "null data pointer");
+    hipStream_t s = sht::as_stream(stream);
+    const sht::StageLayout L = sht::stage_layout(*k, nb_ops);
+    SHT_REQUIRE(L.total <= sht::kStaticStageBytes, SHT_ERR_INVALID,
+                "sht_kernel_call_static: operands of " + std::to_string(nb_ops) + " ops need " +
+                    std::to_string(L.total) + " bytes of staging, above the fixed " +
+                    std::to_string(sht::kStaticStageBytes) + "-byte buffer");
+    sht::kernel_call_staged(*k, a, b, c, nb_ops, s, sht::static_stage_ptr(), L);
   });
 }
 
