@@ -205,6 +205,9 @@ sht_status sht_plan_tasks(sht_plan* p, const int64_t** ta, const int64_t** tb, c
                           const int64_t** kkey, int64_t* ntasks);
 /* useful flops (sum 2*m*n*k) and algorithmic HBM bytes of one execution     */
 sht_status sht_plan_stats(const sht_plan* p, double* flops, double* bytes, int64_t* ntasks, int64_t* ncblocks);
+/* Host wall time (ms) of the plan's build steps: task list and C pattern,
+ * C-stationary tiles, A-stationary schedule (-1 if not built yet).         */
+sht_status sht_plan_build_ms(const sht_plan* p, double* tasks_ms, double* cstat_ms, double* astat_ms);
 sht_status sht_plan_destroy(sht_plan* p);
 
 /* ---- serving: continuous batching of contractions ----------------------

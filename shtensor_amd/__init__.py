@@ -384,6 +384,13 @@ class Plan:
         arr = [np.ctypeslib.as_array(x, shape=(k,)).copy() if k else np.zeros(0, np.int64) for x in ptrs]
         return {"a": arr[0], "b": arr[1], "c": arr[2], "key": arr[3]}
 
+    def build_ms(self):
+        """Host wall time of the build steps (ms): task list + C pattern,
+        C-stationary tiles, A-stationary schedule (-1 if not built)."""
+        t, c, a = C.c_double(), C.c_double(), C.c_double()
+        call("sht_plan_build_ms", C.c_void_p(self.ptr), C.byref(t), C.byref(c), C.byref(a))
+        return {"tasks": t.value, "cstat": c.value, "astat": a.value}
+
     def stats(self):
         f, b = C.c_double(), C.c_double()
         nt, nc = C.c_int64(), C.c_int64()

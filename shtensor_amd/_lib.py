@@ -47,6 +47,7 @@ SIGNATURES = {
     "sht_kernel_create": [C.c_char_p, _PI, _I, _PI, _I, _PI, _I, _D, _D, _I, _I, _PP],
     "sht_kernel_call": [_P, _P, _P, _P, _I64, _P],
     "sht_kernel_call_static": [_P, _P, _P, _P, _I64, _P],
+    "sht_plan_build_ms": [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)],
     "sht_kernel_info": [_P, C.c_char_p, _SZ],
     "sht_kernel_destroy": [_P],
     "sht_kernel_prewarm": [C.c_char_p, _PI, _I, _PI, _I, _PI, _I, _D, _D, _I],

@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <cmath>
@@ -193,6 +194,11 @@ __global__ void k_cblk_recs(TensorDesc C, AxisList crow, AxisList ccol, const in
 }
 
 inline unsigned blocks_for(int64_t n, int tpb = 256) { return static_cast<unsigned>((std::max<int64_t>(n, 1) + tpb - 1) / tpb); }
+
+double wall_ms()
+{
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 int bits_for(uint64_t maxval)
 {
@@ -611,188 +617,370 @@ void build_schedule(sht_plan* P)
   }
 }
 
-// A-stationary schedule from the task list: tasks regrouped by (block row I,
-// contracted key K, block column J).  One panel per contracted key holding
-// the key's partners (B blocks that occur in a task) side by side; one pass
-// per A block, with the C block offset of each panel partner (-1 when the
-// product is not kept); units are row chunks of each block row, ordered
-// XCD-aware.
-void build_as_schedule(sht_plan* P)
+// ---- A-stationary schedule, built on the device --------------------------
+// Per block of a tensor: the keys of its two axis groups (coordinates rolled
+// in the group's axis order) and the element counts over each group.
+__global__ void k_group_keys(TensorDesc d, AxisList r, AxisList c, int64_t* krow, int64_t* kcol, int32_t* erow,
+                             int32_t* ecol)
 {
-  if (P->as_built) return;
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= d.nnz) return;
+  int co[kMaxModes];
+  coords_of(d, d.indices[p], co);
+  int64_t k = 0, mul = 1, e = 1;
+  for (int i = 0; i < r.n; ++i) {
+    k += co[r.ax[i]] * mul;
+    mul *= d.nblk[r.ax[i]];
+    e *= d.bsizes[d.boff[r.ax[i]] + co[r.ax[i]]];
+  }
+  krow[p] = k;
+  erow[p] = static_cast<int32_t>(e);
+  k = 0, mul = 1, e = 1;
+  for (int i = 0; i < c.n; ++i) {
+    k += co[c.ax[i]] * mul;
+    mul *= d.nblk[c.ax[i]];
+    e *= d.bsizes[d.boff[c.ax[i]] + co[c.ax[i]]];
+  }
+  kcol[p] = k;
+  ecol[p] = static_cast<int32_t>(e);
+}
+
+// Sort key of a task in the A-stationary order (block row I, contracted key
+// K, block column J), and the flag of every B block that occurs in a task.
+__global__ void k_as_taskkey(const int64_t* ta, const int64_t* tb, int64_t nt, const int64_t* aI, const int64_t* aK,
+                             const int64_t* bJ, uint64_t nK, uint64_t nJ, uint64_t* key, char* used)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int64_t a = ta[t], b = tb[t];
+  key[t] = (static_cast<uint64_t>(aI[a]) * nK + static_cast<uint64_t>(aK[a])) * nJ + static_cast<uint64_t>(bJ[b]);
+  used[b] = 1;
+}
+
+// First task (in A-stationary order) of every A block.
+__global__ void k_as_segflag(const int64_t* order, const int64_t* ta, int64_t nt, int64_t* flag)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  flag[t] = (t == 0 || ta[order[t]] != ta[order[t - 1]]) ? 1 : 0;
+}
+
+// Per A block (segment s of the sorted tasks): its panel and slot count, and
+// its sweep counts at the two sweep widths.
+__global__ void k_as_segs(const int64_t* order, const int64_t* ta, const int64_t* tb, const int64_t* flag,
+                          const int64_t* incl, int64_t nt, const int32_t* b_panel, const int32_t* pn_nslots,
+                          int64_t* seg_a, int32_t* seg_panel, int64_t* seg_nslots, int64_t* seg_nsw,
+                          int64_t* seg_nsw4)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nt || !flag[t]) return;
+  const int64_t s = incl[t] - 1;
+  const int32_t pn = b_panel[tb[order[t]]];
+  const int64_t ns = pn_nslots[pn];
+  seg_a[s] = ta[order[t]];
+  seg_panel[s] = pn;
+  seg_nslots[s] = ns;
+  seg_nsw[s] = (ns + kAsSweepSlots - 1) / kAsSweepSlots;
+  seg_nsw4[s] = (ns + kAsNarrowSlots - 1) / kAsNarrowSlots;
+}
+
+// The sweep records of every A block at one sweep width (cap slots).
+__global__ void k_as_sweeps(int64_t nseg, const int64_t* seg_a, const int32_t* seg_panel, const int64_t* seg_nslots,
+                            const int64_t* sweep_base, const int64_t* slot_base, const int64_t* a_off,
+                            const int32_t* aM, const int32_t* aK, const int32_t* pn_pitch, const int64_t* pn_bt,
+                            int bf3, int cap, AsSweep* out)
+{
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int64_t a = seg_a[s], ns = seg_nslots[s];
+  const int32_t pn = seg_panel[s];
+  const int32_t pitch = pn_pitch[pn];
+  for (int64_t g = 0, w = sweep_base[s]; g < ns; g += cap, ++w) {
+    AsSweep sw;
+    sw.a_off = a_off[a];
+    sw.b_off = bf3 ? pn_bt[pn] + 16 * g * pitch : pn_bt[pn] + 16 * g;
+    sw.c0 = 16 * (slot_base[s] + g);
+    sw.k = aK[a];
+    sw.m = aM[a];
+    sw.pitch = pitch;
+    sw.ns = static_cast<int32_t>(min<int64_t>(cap, ns - g));
+    out[w] = sw;
+  }
+}
+
+__global__ void k_fill_i64(int64_t* x, int64_t n, int64_t v)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// The C column offsets of every sweep column a task covers: column j of the
+// task's partner inside the panel -> C(0, j) of its C block; marked -(v + 2)
+// when the task is the first (smallest contracted key) of its C block, i.e.
+// the first sweep of the block row that touches the column.
+__global__ void k_as_ccol(const int64_t* order, const int64_t* ta, const int64_t* tb, const int64_t* tc,
+                          const int64_t* incl, int64_t nt, const int64_t* slot_base, const int32_t* aM,
+                          const int32_t* bN, const int32_t* b_col0, const int64_t* c_off, const int64_t* cstart,
+                          int64_t* ccol)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int64_t o = order[t];
+  const int64_t a = ta[o], b = tb[o], c = tc[o];
+  const int64_t base = 16 * slot_base[incl[t] - 1] + b_col0[b];
+  const int64_t M = aM[a], v0 = c_off[c];
+  const bool first = cstart[c] == o;
+  for (int32_t j = 0; j < bN[b]; ++j) {
+    const int64_t v = v0 + j * M;
+    ccol[base + j] = first ? -v - 2 : v;
+  }
+}
+
+__global__ void k_as_rowsegs(int64_t nseg, const int64_t* seg_a, const int64_t* aI, const int32_t* aM,
+                             const int32_t* aK, const int64_t* seg_nslots, int64_t* out_I, int32_t* out_M,
+                             double* out_work)
+{
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int64_t a = seg_a[s];
+  out_I[s] = aI[a];
+  out_M[s] = aM[a];
+  out_work[s] = static_cast<double>(aK[a]) * 16.0 * static_cast<double>(seg_nslots[s]);
+}
+
+__global__ void k_touch(const int64_t* tc, int64_t nt, char* touched)
+{
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < nt) touched[tc[t]] = 1;
+}
+
+// A-stationary schedule from the task list, on the device: tasks sorted by
+// (block row I, contracted key K, block column J) with a radix sort; one panel
+// per contracted key holding the key's partners (B blocks that occur in a
+// task) side by side (panel table on the host: one entry per B block); one
+// sweep list entry per A block and group of cap slots, built from segment
+// scans; the C offset of every sweep column written by one thread per task.
+// The sweep columns of an A block are the same at both sweep widths, so the
+// 8-slot and 4-slot sweep lists share one column array.  Units (row chunks of
+// every block row, dealt XCD-aware) are built on the host from the per-row
+// totals.
+void build_as_schedule_impl(sht_plan* P)
+{
   const int64_t nt = P->ntasks;
   const ContractMaps& m = P->maps;
   const sht_tensor* A = P->A;
   const sht_tensor* B = P->B;
   const sht_tensor* C = P->C;
-  std::vector<int64_t> ta(nt), tb(nt), tc(nt);
-  if (nt) {
-    SHT_HIP(hipMemcpy(ta.data(), P->d_ta, nt * 8, hipMemcpyDeviceToHost));
-    SHT_HIP(hipMemcpy(tb.data(), P->d_tb, nt * 8, hipMemcpyDeviceToHost));
-    SHT_HIP(hipMemcpy(tc.data(), P->d_tc, nt * 8, hipMemcpyDeviceToHost));
-  }
-  auto key = [](const sht_tensor* t, int64_t id, const std::vector<int>& axes) {
-    int64_t k = 0, mul = 1;
-    for (int ax : axes) {
-      k += ((id / t->bstride[ax]) % t->nblk[ax]) * mul;
-      mul *= t->nblk[ax];
-    }
-    return k;
-  };
-  auto elems = [](const sht_tensor* t, int64_t id, const std::vector<int>& axes) {
-    int64_t v = 1;
-    for (int ax : axes) v *= t->bs[ax][(id / t->bstride[ax]) % t->nblk[ax]];
-    return v;
-  };
-  std::vector<int64_t> order(nt);
-  std::iota(order.begin(), order.end(), 0);
-  const size_t na = A->indices.size(), nbk = B->indices.size();
-  std::vector<int64_t> aI(na), aK(na), bK(nbk), bJ(nbk);
-  for (size_t p = 0; p < na; ++p) {
-    aI[p] = key(A, A->indices[p], m.a_row);
-    aK[p] = key(A, A->indices[p], m.a_col);
-  }
-  for (size_t q = 0; q < nbk; ++q) {
-    bK[q] = key(B, B->indices[q], m.b_row);  // same rolled key as aK (contracted axes in A's order)
-    bJ[q] = key(B, B->indices[q], m.b_col);
-  }
-  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
-    const int64_t ix = aI[ta[x]], iy = aI[ta[y]];
-    if (ix != iy) return ix < iy;
-    const int64_t kx = aK[ta[x]], ky = aK[ta[y]];
-    if (kx != ky) return kx < ky;
-    return bJ[tb[x]] < bJ[tb[y]];
-  });
+  Arena& arena = P->ctx->arena;
+  Scratch S(arena);
+  hipStream_t st = nullptr;
   const bool bf3 = P->prec == SHT_PREC_BF16X3;
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
-  // panels: B blocks that occur in a task, grouped by key, ascending J
-  std::vector<int64_t> used;
-  {
-    std::vector<char> seen(nbk, 0);
-    for (int64_t t = 0; t < nt; ++t)
-      if (!seen[tb[t]]) {
-        seen[tb[t]] = 1;
-        used.push_back(tb[t]);
-      }
+  const bool narrow = !exact && !bf3;
+  const int64_t na = static_cast<int64_t>(A->indices.size()), nbk = static_cast<int64_t>(B->indices.size());
+  const TensorDesc dA = A->desc(), dB = B->desc();
+
+  // group keys and element counts of every A and B block
+  int64_t* aI = S.get<int64_t>(na);
+  int64_t* aK = S.get<int64_t>(na);
+  int32_t* aM = S.get<int32_t>(na);
+  int32_t* aKe = S.get<int32_t>(na);
+  int64_t* bK = S.get<int64_t>(nbk);
+  int64_t* bJ = S.get<int64_t>(nbk);
+  int32_t* bKe = S.get<int32_t>(nbk);
+  int32_t* bN = S.get<int32_t>(nbk);
+  if (na)
+    hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(na)), dim3(256), 0, st, dA, axis_list(m.a_row),
+                       axis_list(m.a_col), aI, aK, aM, aKe);
+  if (nbk)
+    hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(nbk)), dim3(256), 0, st, dB, axis_list(m.b_row),
+                       axis_list(m.b_col), bK, bJ, bKe, bN);
+  uint64_t nI = 1, nK = 1, nJ = 1;
+  for (int ax : m.a_row) nI *= static_cast<uint64_t>(A->nblk[ax]);
+  for (int ax : m.a_col) nK *= static_cast<uint64_t>(A->nblk[ax]);
+  for (int ax : m.b_col) nJ *= static_cast<uint64_t>(B->nblk[ax]);
+  SHT_REQUIRE(static_cast<double>(nI) * static_cast<double>(nK) * static_cast<double>(nJ) < 9.0e18,
+              SHT_ERR_UNSUPPORTED, "block index space too large for 64-bit schedule keys");
+
+  // tasks in A-stationary order
+  uint64_t* key = S.get<uint64_t>(nt);
+  uint64_t* key_s = S.get<uint64_t>(nt);
+  int64_t* iota = S.get<int64_t>(nt);
+  int64_t* order = S.get<int64_t>(nt);
+  char* used = S.get<char>(nbk);
+  SHT_HIP(hipMemsetAsync(used, 0, std::max<int64_t>(nbk, 1), st));
+  if (nt) {
+    hipLaunchKernelGGL(k_as_taskkey, dim3(blocks_for(nt)), dim3(256), 0, st, P->d_ta, P->d_tb, nt, aI, aK, bJ, nK, nJ,
+                       key, used);
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(nt)), dim3(256), 0, st, iota, nt);
+    size_t tmp = 0;
+    const int bits = bits_for(nI * nK * nJ);
+    SHT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key_s, iota, order, static_cast<int>(nt), 0, bits,
+                                               st));
+    SHT_HIP(hipcub::DeviceRadixSort::SortPairs(S.bytes(tmp), tmp, key, key_s, iota, order, static_cast<int>(nt), 0,
+                                               bits, st));
   }
-  std::sort(used.begin(), used.end(), [&](int64_t x, int64_t y) {
-    if (bK[x] != bK[y]) return bK[x] < bK[y];
-    return bJ[x] < bJ[y];
+  SHT_HIP(hipGetLastError());
+
+  // panels (host table over the B blocks): the used B blocks of each
+  // contracted key in ascending block column, side by side
+  std::vector<char> h_used(static_cast<size_t>(nbk));
+  std::vector<int64_t> hbK(nbk), hbJ(nbk);
+  std::vector<int32_t> hbKe(nbk), hbN(nbk);
+  if (nbk) {
+    SHT_HIP(hipMemcpy(h_used.data(), used, nbk, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(hbK.data(), bK, nbk * 8, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(hbJ.data(), bJ, nbk * 8, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(hbKe.data(), bKe, nbk * 4, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(hbN.data(), bN, nbk * 4, hipMemcpyDeviceToHost));
+  }
+  std::vector<int64_t> usedq;
+  for (int64_t q = 0; q < nbk; ++q)
+    if (h_used[q]) usedq.push_back(q);
+  std::sort(usedq.begin(), usedq.end(), [&](int64_t x, int64_t y) {
+    if (hbK[x] != hbK[y]) return hbK[x] < hbK[y];
+    return hbJ[x] < hbJ[y];
   });
-  struct AsCol {  // panel column -> (partner index within the panel, column of the partner)
-    int32_t p, j;
-  };
-  struct Panel {
-    int64_t bt_off;
-    int32_t k, ncol, pitch, col0, npart;
-  };
-  std::vector<Panel> panels;
-  std::vector<int32_t> panel_of(nbk, -1), part_of(nbk, -1);
+  std::vector<int32_t> b_panel(nbk, -1), b_col0(nbk, 0);
+  std::vector<int32_t> pn_nslots, pn_pitch;
+  std::vector<int64_t> pn_bt;
   std::vector<BtRec> bt;
-  std::vector<AsCol> cols;
   int64_t bt_total = 0;
-  for (size_t u = 0; u < used.size();) {
+  for (size_t u = 0; u < usedq.size();) {
     size_t e = u;
-    while (e < used.size() && bK[used[e]] == bK[used[u]]) ++e;
-    Panel pn{};
-    pn.k = static_cast<int32_t>(elems(B, B->indices[used[u]], m.b_row));
-    pn.col0 = static_cast<int32_t>(cols.size());
+    while (e < usedq.size() && hbK[usedq[e]] == hbK[usedq[u]]) ++e;
+    const int32_t pk = hbKe[usedq[u]];
     int64_t ncol = 0;
-    for (size_t x = u; x < e; ++x) {
-      const int64_t q = used[x];
-      const int32_t n = static_cast<int32_t>(elems(B, B->indices[q], m.b_col));
-      panel_of[q] = static_cast<int32_t>(panels.size());
-      part_of[q] = static_cast<int32_t>(x - u);
-      for (int32_t j = 0; j < n; ++j) cols.push_back(AsCol{static_cast<int32_t>(x - u), j});
-      ncol += n;
-    }
+    for (size_t x = u; x < e; ++x) ncol += hbN[usedq[x]];
     SHT_REQUIRE(ncol < (int64_t(1) << 30), SHT_ERR_UNSUPPORTED, "panel too wide");
-    pn.ncol = static_cast<int32_t>(ncol);
-    pn.npart = static_cast<int32_t>(e - u);
     const int64_t npad = (ncol + kBtPad - 1) / kBtPad * kBtPad;
-    pn.bt_off = bt_total;
+    const int32_t pitch = bf3 ? (pk + kBfK - 1) / kBfK * kBfK : static_cast<int32_t>(npad);
+    const int32_t id = static_cast<int32_t>(pn_nslots.size());
     int64_t c = 0;
-    if (bf3) {
-      pn.pitch = (pn.k + kBfK - 1) / kBfK * kBfK;
-      for (size_t x = u; x < e; ++x) {
-        const int64_t q = used[x];
-        const int32_t n = static_cast<int32_t>(elems(B, B->indices[q], m.b_col));
-        bt.push_back(BtRec{B->offsets[q], pn.bt_off + c * pn.pitch, pn.k, n, pn.pitch, 0});
-        c += n;
-      }
-      bt_total += npad * pn.pitch;
-    } else {
-      pn.pitch = static_cast<int32_t>(npad);
-      for (size_t x = u; x < e; ++x) {
-        const int64_t q = used[x];
-        const int32_t n = static_cast<int32_t>(elems(B, B->indices[q], m.b_col));
-        bt.push_back(BtRec{B->offsets[q], pn.bt_off + c, pn.k, n, pn.pitch, 0});
-        c += n;
-      }
-      bt_total += static_cast<int64_t>((pn.k + 15) / 16 * 16) * npad;
+    for (size_t x = u; x < e; ++x) {
+      const int64_t q = usedq[x];
+      b_panel[q] = id;
+      b_col0[q] = static_cast<int32_t>(c);
+      bt.push_back(BtRec{B->offsets[q], bf3 ? bt_total + c * pitch : bt_total + c, pk, hbN[q], pitch, 0});
+      c += hbN[q];
     }
-    panels.push_back(pn);
+    pn_nslots.push_back(static_cast<int32_t>(npad / 16));
+    pn_pitch.push_back(pitch);
+    pn_bt.push_back(bt_total);
+    bt_total += bf3 ? npad * pitch : static_cast<int64_t>((pk + 15) / 16 * 16) * npad;
     u = e;
   }
-  // sweep list per block row (sweeps of at most `cap` slots), with the C
-  // offset of every sweep column
+  const int64_t npanel = static_cast<int64_t>(pn_nslots.size());
+  auto up = [&](const void* src, size_t bytes) {
+    void* d = S.bytes(std::max<size_t>(bytes, 1));
+    if (bytes) SHT_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    return d;
+  };
+  const int32_t* d_b_panel = static_cast<const int32_t*>(up(b_panel.data(), nbk * 4));
+  const int32_t* d_b_col0 = static_cast<const int32_t*>(up(b_col0.data(), nbk * 4));
+  const int32_t* d_pn_nslots = static_cast<const int32_t*>(up(pn_nslots.data(), npanel * 4));
+  const int32_t* d_pn_pitch = static_cast<const int32_t*>(up(pn_pitch.data(), npanel * 4));
+  const int64_t* d_pn_bt = static_cast<const int64_t*>(up(pn_bt.data(), npanel * 8));
+
+  // A blocks = segments of the sorted tasks; sweep and slot offsets by scans
+  int64_t* flag = S.get<int64_t>(nt);
+  int64_t* incl = S.get<int64_t>(nt);
+  int64_t nseg = 0;
+  if (nt) {
+    hipLaunchKernelGGL(k_as_segflag, dim3(blocks_for(nt)), dim3(256), 0, st, order, P->d_ta, nt, flag);
+    size_t tmp = 0;
+    SHT_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, flag, incl, static_cast<int>(nt), st));
+    SHT_HIP(hipcub::DeviceScan::InclusiveSum(S.bytes(tmp), tmp, flag, incl, static_cast<int>(nt), st));
+    SHT_HIP(hipMemcpy(&nseg, incl + nt - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+  int64_t* seg_a = S.get<int64_t>(nseg);
+  int32_t* seg_panel = S.get<int32_t>(nseg);
+  int64_t* seg_nslots = S.get<int64_t>(nseg);
+  int64_t* seg_nsw = S.get<int64_t>(nseg);
+  int64_t* seg_nsw4 = S.get<int64_t>(nseg);
+  int64_t* slot_base = S.get<int64_t>(nseg);
+  int64_t* sweep_base = S.get<int64_t>(nseg);
+  int64_t* sweep_base4 = S.get<int64_t>(nseg);
+  int64_t nsweep = 0, nsweep4 = 0, nslot = 0;
+  if (nseg) {
+    hipLaunchKernelGGL(k_as_segs, dim3(blocks_for(nt)), dim3(256), 0, st, order, P->d_ta, P->d_tb, flag, incl, nt,
+                       d_b_panel, d_pn_nslots, seg_a, seg_panel, seg_nslots, seg_nsw, seg_nsw4);
+    auto excl = [&](const int64_t* in, int64_t* out) {
+      size_t tmp = 0;
+      SHT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, static_cast<int>(nseg), st));
+      SHT_HIP(hipcub::DeviceScan::ExclusiveSum(S.bytes(tmp), tmp, in, out, static_cast<int>(nseg), st));
+      int64_t lo = 0, lc = 0;
+      SHT_HIP(hipMemcpy(&lo, out + nseg - 1, 8, hipMemcpyDeviceToHost));
+      SHT_HIP(hipMemcpy(&lc, in + nseg - 1, 8, hipMemcpyDeviceToHost));
+      return lo + lc;
+    };
+    nslot = excl(seg_nslots, slot_base);
+    nsweep = excl(seg_nsw, sweep_base);
+    if (narrow) nsweep4 = excl(seg_nsw4, sweep_base4);
+  }
+  SHT_REQUIRE(nsweep < (int64_t(1) << 31) && nsweep4 < (int64_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
+
+  // sweep lists and their shared column offsets (kept by the plan)
+  int64_t* a_off = A->d_offsets;
+  P->d_as_sweep = static_cast<AsSweep*>(arena.allocate(std::max<int64_t>(nsweep, 1) * sizeof(AsSweep)));
+  P->d_as_ccol = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(16 * nslot, 1) * sizeof(int64_t)));
+  if (narrow) P->d_as_sweep4 = static_cast<AsSweep*>(arena.allocate(std::max<int64_t>(nsweep4, 1) * sizeof(AsSweep)));
+  if (nseg) {
+    hipLaunchKernelGGL(k_as_sweeps, dim3(blocks_for(nseg)), dim3(256), 0, st, nseg, seg_a, seg_panel, seg_nslots,
+                       sweep_base, slot_base, a_off, aM, aKe, d_pn_pitch, d_pn_bt, bf3 ? 1 : 0, kAsSweepSlots,
+                       P->d_as_sweep);
+    if (narrow)
+      hipLaunchKernelGGL(k_as_sweeps, dim3(blocks_for(nseg)), dim3(256), 0, st, nseg, seg_a, seg_panel, seg_nslots,
+                         sweep_base4, slot_base, a_off, aM, aKe, d_pn_pitch, d_pn_bt, 0, kAsNarrowSlots,
+                         P->d_as_sweep4);
+    hipLaunchKernelGGL(k_fill_i64, dim3(blocks_for(16 * nslot)), dim3(256), 0, st, P->d_as_ccol, 16 * nslot,
+                       int64_t(-1));
+    // first task of every C block in the task list (sorted by C position)
+    int64_t* cstart = S.get<int64_t>(P->ncb + 1);
+    hipLaunchKernelGGL(k_cstart, dim3(blocks_for(P->ncb + 1)), dim3(256), 0, st, P->d_tc, nt, P->ncb, cstart);
+    hipLaunchKernelGGL(k_as_ccol, dim3(blocks_for(nt)), dim3(256), 0, st, order, P->d_ta, P->d_tb, P->d_tc, incl, nt,
+                       slot_base, aM, bN, d_b_col0, C->d_offsets, cstart, P->d_as_ccol);
+  }
+  SHT_HIP(hipGetLastError());
+
+  // block rows: runs of A blocks with the same row key (host, per segment)
+  std::vector<int64_t> sI(nseg), snsw(nseg), snsw4(nseg);
+  std::vector<int32_t> sM(nseg);
+  std::vector<double> swork(nseg);
+  if (nseg) {
+    int64_t* oI = S.get<int64_t>(nseg);
+    int32_t* oM = S.get<int32_t>(nseg);
+    double* ow = S.get<double>(nseg);
+    hipLaunchKernelGGL(k_as_rowsegs, dim3(blocks_for(nseg)), dim3(256), 0, st, nseg, seg_a, aI, aM, aKe, seg_nslots,
+                       oI, oM, ow);
+    SHT_HIP(hipMemcpy(sI.data(), oI, nseg * 8, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(sM.data(), oM, nseg * 4, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(swork.data(), ow, nseg * 8, hipMemcpyDeviceToHost));
+    SHT_HIP(hipMemcpy(snsw.data(), seg_nsw, nseg * 8, hipMemcpyDeviceToHost));
+    if (narrow) SHT_HIP(hipMemcpy(snsw4.data(), seg_nsw4, nseg * 8, hipMemcpyDeviceToHost));
+  }
   struct Row {
     int32_t sweep0, nsweep;
     int64_t rows;
     double work;  // sum over sweeps of k * columns
   };
-  auto gen_sweeps = [&](int cap, std::vector<AsSweep>& sweeps, std::vector<int64_t>& ccol, std::vector<Row>& brows) {
-    std::vector<int64_t> pcof;  // partner C offsets of the current A block
-    std::unordered_set<int64_t> seen;  // C columns of the block row touched by an earlier sweep
-    int64_t t = 0;
-    while (t < nt) {
-      const int64_t I = aI[ta[order[t]]];
-      seen.clear();
-      Row row{static_cast<int32_t>(sweeps.size()), 0, elems(A, A->indices[ta[order[t]]], m.a_row), 0.0};
-      while (t < nt && aI[ta[order[t]]] == I) {
-        const int64_t a = ta[order[t]];
-        const Panel& pn = panels[panel_of[tb[order[t]]]];
-        pcof.assign(pn.npart, -1);
-        while (t < nt && ta[order[t]] == a) {
-          pcof[part_of[tb[order[t]]]] = C->offsets[tc[order[t]]];
-          ++t;
-        }
-        const int32_t k = static_cast<int32_t>(elems(A, A->indices[a], m.a_col));
-        const int nslots = (pn.ncol + kBtPad - 1) / kBtPad;
-        for (int g = 0; g < nslots; g += cap) {
-          AsSweep sw{};
-          sw.a_off = A->offsets[a];
-          sw.b_off = bf3 ? pn.bt_off + static_cast<int64_t>(16 * g) * pn.pitch : pn.bt_off + 16 * g;
-          sw.c0 = static_cast<int64_t>(ccol.size());
-          sw.k = k;
-          sw.m = static_cast<int32_t>(row.rows);
-          sw.pitch = pn.pitch;
-          sw.ns = std::min(cap, nslots - g);
-          for (int c = 16 * g; c < 16 * (g + sw.ns); ++c) {
-            int64_t v = -1;
-            if (c < pn.ncol) {
-              const AsCol& cm = cols[pn.col0 + c];
-              if (pcof[cm.p] >= 0) v = pcof[cm.p] + static_cast<int64_t>(cm.j) * row.rows;
-            }
-            if (v >= 0 && seen.insert(v).second) v = -v - 2;  // first touch of the column
-            ccol.push_back(v);
-          }
-          row.work += static_cast<double>(k) * 16 * sw.ns;
-          sweeps.push_back(sw);
-        }
+  auto rows_of = [&](const std::vector<int64_t>& nsw) {
+    std::vector<Row> out;
+    int64_t w = 0;
+    for (int64_t s = 0; s < nseg;) {
+      Row r{static_cast<int32_t>(w), 0, sM[s], 0.0};
+      const int64_t I = sI[s];
+      for (; s < nseg && sI[s] == I; ++s) {
+        r.nsweep += static_cast<int32_t>(nsw[s]);
+        r.work += swork[s];
+        w += nsw[s];
       }
-      row.nsweep = static_cast<int32_t>(sweeps.size()) - row.sweep0;
-      brows.push_back(row);
+      out.push_back(r);
     }
-    SHT_REQUIRE(sweeps.size() < (size_t(1) << 31), SHT_ERR_UNSUPPORTED, "schedule too large");
+    return out;
   };
-  std::vector<AsSweep> sweeps, sweeps4;
-  std::vector<int64_t> ccol, ccol4;
-  std::vector<Row> brows, brows4;
-  gen_sweeps(kAsSweepSlots, sweeps, ccol, brows);
-  // narrow sweeps (schedule 3): FP64 / FP32 MFMA only
-  const bool narrow = !exact && !bf3;
-  if (narrow) gen_sweeps(kAsNarrowSlots, sweeps4, ccol4, brows4);
+  const std::vector<Row> brows = rows_of(snsw);
+  const std::vector<Row> brows4 = narrow ? rows_of(snsw4) : std::vector<Row>();
   // XCD-aware order.  Workgroup g runs on XCD g % 8 (round-robin dispatch),
   // and every XCD has its own L2: all units of a block row go to one XCD, so
   // the row's panels are shared in that L2.  Rows are dealt to the XCD with
@@ -803,16 +991,16 @@ void build_as_schedule(sht_plan* P)
   // ghost units, so a workgroup never mixes rows): the waves of a workgroup
   // run the same sweep list on neighbouring row chunks and share the panel
   // fragments (through LDS for the FP64/FP32 classes, L1 for split-BF16).
-  auto build = [&](const std::vector<Row>& brows, int step, int per_wg) {
-    std::vector<int64_t> ro(brows.size());
+  auto build = [&](const std::vector<Row>& rows, int step, int per_wg) {
+    std::vector<int64_t> ro(rows.size());
     std::iota(ro.begin(), ro.end(), 0);
     std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) {
-      return brows[x].work * brows[x].rows > brows[y].work * brows[y].rows;
+      return rows[x].work * rows[x].rows > rows[y].work * rows[y].rows;
     });
     std::vector<std::vector<AsUnit>> q(kXcd);
     std::vector<double> load(kXcd, 0.0);
     for (int64_t r : ro) {
-      const Row& br = brows[r];
+      const Row& br = rows[r];
       const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
       for (int64_t r0 = 0; r0 < br.rows; r0 += step)
         q[x].push_back(AsUnit{br.sweep0, br.nsweep, static_cast<int32_t>(r0), 0});
@@ -836,44 +1024,49 @@ void build_as_schedule(sht_plan* P)
   std::vector<AsUnit> us = build(brows, exact ? kAsUnitRowsW : kAsUnitRows, 1);
   std::vector<AsUnit> usw = exact ? std::vector<AsUnit>() : build(brows, kAsUnitRows, kAsGroupWaves);
   std::vector<AsUnit> us4 = narrow ? build(brows4, kAsUnitRowsN, 1) : std::vector<AsUnit>();
-  Arena& arena = P->ctx->arena;
-  auto upload = [&](const void* src, size_t bytes) {
+  auto keep = [&](const void* src, size_t bytes) {
     void* d = arena.allocate(std::max<size_t>(bytes, 1));
     if (bytes) SHT_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
     return d;
   };
-  P->d_as_sweep = static_cast<AsSweep*>(upload(sweeps.data(), sweeps.size() * sizeof(AsSweep)));
-  P->d_as_ccol = static_cast<int64_t*>(upload(ccol.data(), ccol.size() * sizeof(int64_t)));
-  P->d_as_unit = static_cast<AsUnit*>(upload(us.data(), us.size() * sizeof(AsUnit)));
-  P->d_as_unitw = static_cast<AsUnit*>(upload(usw.data(), usw.size() * sizeof(AsUnit)));
-  if (narrow) {
-    P->d_as_sweep4 = static_cast<AsSweep*>(upload(sweeps4.data(), sweeps4.size() * sizeof(AsSweep)));
-    P->d_as_ccol4 = static_cast<int64_t*>(upload(ccol4.data(), ccol4.size() * sizeof(int64_t)));
-    P->d_as_unit4 = static_cast<AsUnit*>(upload(us4.data(), us4.size() * sizeof(AsUnit)));
-  }
-  P->d_bt_rec = static_cast<BtRec*>(upload(bt.data(), bt.size() * sizeof(BtRec)));
+  P->d_as_unit = static_cast<AsUnit*>(keep(us.data(), us.size() * sizeof(AsUnit)));
+  P->d_as_unitw = static_cast<AsUnit*>(keep(usw.data(), usw.size() * sizeof(AsUnit)));
+  if (narrow) P->d_as_unit4 = static_cast<AsUnit*>(keep(us4.data(), us4.size() * sizeof(AsUnit)));
+  P->d_bt_rec = static_cast<BtRec*>(keep(bt.data(), bt.size() * sizeof(BtRec)));
   const size_t bt_bytes = bf3 ? 3 * sizeof(uint16_t) : (P->prec == SHT_PREC_FP32 ? sizeof(float) : sizeof(double));
   P->d_bt = arena.allocate(std::max<int64_t>(bt_total, 1) * bt_bytes);
   // pad columns and rows of the panels stay zero: the conversion writes only
   // the blocks' own columns
-  SHT_HIP(hipMemset(P->d_bt, 0, std::max<int64_t>(bt_total, 1) * bt_bytes));
-  SHT_HIP(hipDeviceSynchronize());
+  SHT_HIP(hipMemsetAsync(P->d_bt, 0, std::max<int64_t>(bt_total, 1) * bt_bytes, st));
   {
-    std::vector<char> touched(C->indices.size(), 0);
-    for (int64_t x = 0; x < nt; ++x) touched[tc[x]] = 1;
-    P->as_all_touched = std::all_of(touched.begin(), touched.end(), [](char c) { return c != 0; });
+    // every C block has a task: first sweeps may start from zeros
+    const int64_t ncb = static_cast<int64_t>(C->indices.size());
+    char* touched = S.get<char>(ncb);
+    SHT_HIP(hipMemsetAsync(touched, 0, std::max<int64_t>(ncb, 1), st));
+    if (nt) hipLaunchKernelGGL(k_touch, dim3(blocks_for(nt)), dim3(256), 0, st, P->d_tc, nt, touched);
+    std::vector<char> h(static_cast<size_t>(ncb));
+    if (ncb) SHT_HIP(hipMemcpy(h.data(), touched, ncb, hipMemcpyDeviceToHost));
+    P->as_all_touched = std::all_of(h.begin(), h.end(), [](char c) { return c != 0; });
     P->d_as_zero = static_cast<double*>(arena.allocate(kAsZeroLen * sizeof(double)));
-    SHT_HIP(hipMemset(P->d_as_zero, 0, kAsZeroLen * sizeof(double)));
-    SHT_HIP(hipDeviceSynchronize());
+    SHT_HIP(hipMemsetAsync(P->d_as_zero, 0, kAsZeroLen * sizeof(double), st));
   }
+  SHT_HIP(hipStreamSynchronize(st));
   P->bt_total = bt_total;
   P->bt_nrec = static_cast<int64_t>(bt.size());
   P->as_nunits = static_cast<int64_t>(us.size());
   P->as_nunitsw = static_cast<int64_t>(usw.size());
   P->as_nunits4 = static_cast<int64_t>(us4.size());
-  P->as_nsweep = static_cast<int64_t>(sweeps.size());
-  P->as_npanel = static_cast<int64_t>(panels.size());
+  P->as_nsweep = nsweep;
+  P->as_npanel = npanel;
   P->as_built = true;
+}
+
+void build_as_schedule(sht_plan* P)
+{
+  if (P->as_built) return;
+  const double t0 = wall_ms();
+  build_as_schedule_impl(P);
+  P->ms_astat = wall_ms() - t0;
 }
 
 // The A-stationary kernels address an A block with 32-bit element offsets:
@@ -987,7 +1180,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       const AsClass cls = as_class(P);
       const bool wide = as_group_waves(cls) > 1, narrow = P->schedule == 3;
       g.sweeps = narrow ? P->d_as_sweep4 : P->d_as_sweep;
-      g.ccol = narrow ? P->d_as_ccol4 : P->d_as_ccol;
+      g.ccol = P->d_as_ccol;  // (both sweep widths share the column offsets)
       g.units = narrow ? P->d_as_unit4 : (wide ? P->d_as_unitw : P->d_as_unit);
       g.nunits = narrow ? P->as_nunits4 : (wide ? P->as_nunitsw : P->as_nunits);
       g.alpha = alpha;
@@ -1053,8 +1246,12 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
     for (size_t r = 0; r < m.a_col.size(); ++r)
       SHT_REQUIRE(A->bs[m.a_col[r]] == B->bs[m.b_row[r]], SHT_ERR_INVALID, "Block sizes of A and B do not match");
     try {
+      const double t0 = sht::wall_ms();
       sht::build_tasks(P.get());
+      const double t1 = sht::wall_ms();
       sht::build_schedule(P.get());
+      P->ms_cstat = sht::wall_ms() - t1;
+      P->ms_tasks = t1 - t0;
       sht::setup_staging(P.get());
     } catch (...) {
       P->release();
@@ -1268,6 +1465,16 @@ sht_status sht_plan_stats(const sht_plan* P, double* flops, double* bytes, int64
     if (bytes) *bytes = P->bytes_in;
     if (ntasks) *ntasks = P->ntasks;
     if (ncblocks) *ncblocks = P->ncb;
+  });
+}
+
+sht_status sht_plan_build_ms(const sht_plan* P, double* tasks_ms, double* cstat_ms, double* astat_ms)
+{
+  return guard([&] {
+    SHT_REQUIRE(P, SHT_ERR_INVALID, "null plan");
+    if (tasks_ms) *tasks_ms = P->ms_tasks;
+    if (cstat_ms) *cstat_ms = P->ms_cstat;
+    if (astat_ms) *astat_ms = P->as_built ? P->ms_astat : -1.0;
   });
 }
 

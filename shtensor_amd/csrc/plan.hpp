@@ -215,8 +215,7 @@ struct sht_plan {
   int64_t* d_as_ccol = nullptr;
   sht::AsUnit* d_as_unit = nullptr;   // one unit per workgroup (32 rows; 64 for the fma chain)
   sht::AsUnit* d_as_unitw = nullptr;  // kAsGroupWaves units of one block row per workgroup
-  sht::AsSweep* d_as_sweep4 = nullptr;  // the narrow sweep list (kAsNarrowSlots), its columns and units
-  int64_t* d_as_ccol4 = nullptr;
+  sht::AsSweep* d_as_sweep4 = nullptr;  // the narrow sweep list (kAsNarrowSlots; columns shared) and units
   sht::AsUnit* d_as_unit4 = nullptr;
   sht::BtRec* d_bt_rec = nullptr;
   void* d_bt = nullptr;
@@ -224,6 +223,9 @@ struct sht_plan {
   int64_t as_nunits = 0, as_nunitsw = 0, as_nunits4 = 0, as_nsweep = 0, as_npanel = 0;
   bool as_all_touched = false;  // every C block has a task (first sweeps may initialise C)
   double* d_as_zero = nullptr;  // kAsZeroLen zeros
+  // host wall time of the build steps (task list + C pattern, C-stationary
+  // tiles, A-stationary schedule), ms
+  double ms_tasks = 0.0, ms_cstat = 0.0, ms_astat = 0.0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   double graph_alpha = 0.0, graph_beta = 0.0;
@@ -235,7 +237,7 @@ struct sht_plan {
                     static_cast<void*>(d_kk), static_cast<void*>(d_tasks), static_cast<void*>(d_cblk),
                     static_cast<void*>(d_tiles), d_Ap, d_Bp, d_Cp, static_cast<void*>(d_as_sweep),
                     static_cast<void*>(d_as_ccol), static_cast<void*>(d_as_unit),
-                    static_cast<void*>(d_as_unitw), static_cast<void*>(d_as_sweep4), static_cast<void*>(d_as_ccol4),
+                    static_cast<void*>(d_as_unitw), static_cast<void*>(d_as_sweep4),
                     static_cast<void*>(d_as_unit4), static_cast<void*>(d_bt_rec), d_bt,
                     static_cast<void*>(d_as_zero)})
       if (p) a.release(p);
