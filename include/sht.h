@@ -27,7 +27,8 @@ typedef enum {
   SHT_ERR_HIP = 4,         /* HIP runtime or hipRTC failure                     */
   SHT_ERR_STATE = 5,       /* object in the wrong state (e.g. not reserved)     */
   SHT_ERR_UNSUPPORTED = 6, /* valid request this build does not implement       */
-  SHT_ERR_ADMISSION = 7    /* serving: request over the admission budget        */
+  SHT_ERR_ADMISSION = 7,   /* serving: request over the admission budget        */
+  SHT_ERR_CANCELLED = 8    /* serving: request cancelled before it was launched  */
 } sht_status;
 
 /* src/Basics/Definitions.h:45-49 */
@@ -189,7 +190,9 @@ const char* sht_kclass_name(int kclass);
  * (not for SHT_PREC_FP64_EXACT); 3 = schedule 1 over sweeps of at most 4
  * slots at three waves per SIMD (SHT_PREC_FP64 and SHT_PREC_FP32 only).
  * Schedules 1-3 need every A block below 2^28 elements.  Results do not
- * depend on the schedule (FP64 forms: bit for bit).                         */
+ * depend on the schedule (FP64 forms: bit for bit).  A new plan starts with
+ * schedule 2 when A holds at least 8x B's elements and its tasks average a
+ * contracted length of at least 64 (not SHT_PREC_FP64_EXACT), else 0.     */
 sht_status sht_plan_set_schedule(sht_plan* p, int schedule);
 sht_status sht_plan_get_schedule(const sht_plan* p, int* schedule);
 /* Autotuner: time each applicable schedule (`reps` executions on `stream`)
@@ -213,21 +216,38 @@ sht_status sht_plan_destroy(sht_plan* p);
 /* ---- serving: continuous batching of contractions ----------------------
  * Requests name operand tensors resident in the context's arena.  A
  * dispatcher thread admits requests in arrival order while the summed
- * footprint (A + B + reserved C bytes) of unfinished requests fits
- * `budget_bytes`, plans them, and merges the tile lists of all ready
- * requests with equal (precision, alpha, beta) into shared grouped launches
- * on its own stream.  A request larger than the budget is rejected with
- * SHT_ERR_ADMISSION.  C must not be touched until sht_server_wait returns.
- * Results are bit-identical to a stand-alone sht_contract.                  */
+ * footprint of unfinished requests fits `budget_bytes` (A + B + reserved C +
+ * the permuted staging copies the expression needs at submit; corrected to
+ * the operands, C and the plan's own memory once planned); a request that
+ * writes a tensor an earlier unfinished request touches, or reads one it
+ * writes, waits for it, while later independent requests pass it.  Planner
+ * threads build the plans (a plan that runs out of arena while other
+ * requests hold memory is queued again).  Ready requests with C-stationary
+ * plans and equal (precision, alpha, beta) are merged into shared grouped
+ * launches; a request whose plan picked the A-stationary schedule runs its
+ * own launch.  Batches are dealt to 4 device streams, so work admitted while
+ * other batches run overlaps them on the GPU.  A request larger than the
+ * budget is rejected with SHT_ERR_ADMISSION.  C must not be touched until
+ * sht_server_wait returns.  Results are bit-identical to a stand-alone
+ * sht_contract.                                                             */
 typedef struct sht_server sht_server;
 sht_status sht_server_create(sht_ctx* ctx, size_t budget_bytes, int max_batch, sht_server** out);
 sht_status sht_server_submit(sht_server* s, const char* expr, double alpha, const sht_tensor* A, const sht_tensor* B,
                              double beta, sht_tensor* C, sht_precision prec, int64_t* req_id);
-/* blocks until the request finished; latency = completion - submission     */
+/* blocks until the request finished; latency = completion - submission.
+ * A cancelled request returns SHT_ERR_CANCELLED.                            */
 sht_status sht_server_wait(sht_server* s, int64_t req_id, double* latency_ms);
 sht_status sht_server_poll(sht_server* s, int64_t req_id, int* done);
+/* Cancel a request that has not been launched (waiting for admission, being
+ * planned or ready): it completes with SHT_ERR_CANCELLED (a request being
+ * planned may already have reserved C's pattern).  SHT_ERR_STATE once it is
+ * on the device or finished.                                                */
+sht_status sht_server_cancel(sht_server* s, int64_t req_id);
 sht_status sht_server_stats(sht_server* s, int64_t* nreq_done, int64_t* nbatches, int64_t* max_merged,
                             int64_t* nlaunches);
+/* cancelled and re-queued (arena exhausted at plan time) request counts, and
+ * the bytes currently counted against the budget                            */
+sht_status sht_server_counters(sht_server* s, int64_t* ncancelled, int64_t* nrequeued, size_t* in_use);
 sht_status sht_server_destroy(sht_server* s);
 
 #ifdef __cplusplus

@@ -436,6 +436,16 @@ class Server:
         call("sht_server_poll", C.c_void_p(self.ptr), int(rid), C.byref(d))
         return bool(d.value)
 
+    def cancel(self, rid):
+        """Cancel a request not yet launched (its wait then raises ShtError
+        with code ERR_CANCELLED); raises ShtError(ERR_STATE) once launched."""
+        call("sht_server_cancel", C.c_void_p(self.ptr), int(rid))
+
+    def counters(self):
+        nc, nr, iu = C.c_int64(), C.c_int64(), C.c_size_t()
+        call("sht_server_counters", C.c_void_p(self.ptr), C.byref(nc), C.byref(nr), C.byref(iu))
+        return {"cancelled": nc.value, "requeued": nr.value, "in_use": iu.value}
+
     def stats(self):
         v = [C.c_int64() for _ in range(4)]
         call("sht_server_stats", C.c_void_p(self.ptr), *[C.byref(x) for x in v])

@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libshtensor_amd.so")
 
-OK, ERR_INVALID, ERR_EXPR, ERR_OOM, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ADMISSION = range(8)
+OK, ERR_INVALID, ERR_EXPR, ERR_OOM, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_ADMISSION, ERR_CANCELLED = range(9)
 FLOAT32, FLOAT64 = 0, 1
 LAPACK, XMM = 0, 1
 FROBENIUS, MAXABS = 0, 1
@@ -48,6 +48,8 @@ SIGNATURES = {
     "sht_kernel_call": [_P, _P, _P, _P, _I64, _P],
     "sht_kernel_call_static": [_P, _P, _P, _P, _I64, _P],
     "sht_plan_build_ms": [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)],
+    "sht_server_cancel": [_P, _I64],
+    "sht_server_counters": [_P, _PI64, _PI64, C.POINTER(_SZ)],
     "sht_kernel_info": [_P, C.c_char_p, _SZ],
     "sht_kernel_destroy": [_P],
     "sht_kernel_prewarm": [C.c_char_p, _PI, _I, _PI, _I, _PI, _I, _D, _D, _I],

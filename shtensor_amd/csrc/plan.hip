@@ -494,6 +494,9 @@ void build_schedule(sht_plan* P)
   double fl = 0.0;
   for (int64_t c = 0; c < ncb; ++c) fl += 2.0 * cb[c].m * static_cast<double>(cb[c].n) * static_cast<double>(w[c]);
   P->flops = fl;
+  double sk = 0.0;
+  for (int64_t c = 0; c < ncb; ++c) sk += static_cast<double>(w[c]);
+  P->sum_k = sk;
   const double es = 8.0;
   P->bytes_in = es * (static_cast<double>(P->A->nelem) + static_cast<double>(P->B->nelem) +
                       static_cast<double>(P->C->nelem));
@@ -1102,6 +1105,23 @@ AsClass as_class(const sht_plan* P)
 // schedule 3 (narrow sweeps) exists for the FP64 and FP32 MFMA forms
 bool narrow_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32; }
 
+// The schedule a new plan starts with (sht_plan_autotune measures instead):
+// the grouped A-stationary form (2) where it pays -- A at least 8x B, so A
+// dominates the traffic and is read once, and tasks with contracted lengths
+// of 64 or more on average, so each sweep's k loop is long against its C
+// tile load and store (config 1: 186 on average, 52 ms against 94 ms
+// C-stationary); otherwise C-stationary tiles (0), which win for short k
+// (configs 0, 2: 16 and 14) and for A and B alike (config 3).  The fma-chain
+// form stays C-stationary (its A-stationary kernel is slower at every
+// measured size).
+int default_schedule(const sht_plan* P)
+{
+  if (P->prec == SHT_PREC_FP64_EXACT || P->ntasks == 0 || !as_supported(P)) return 0;
+  const double mean_k = P->sum_k / static_cast<double>(P->ntasks);
+  const bool a_dominant = static_cast<double>(P->A->nelem) >= 8.0 * static_cast<double>(P->B->nelem);
+  return a_dominant && mean_k >= 64.0 ? 2 : 0;
+}
+
 void setup_staging(sht_plan* P)
 {
   const ContractMaps& m = P->maps;
@@ -1253,6 +1273,8 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
       P->ms_cstat = sht::wall_ms() - t1;
       P->ms_tasks = t1 - t0;
       sht::setup_staging(P.get());
+      P->schedule = sht::default_schedule(P.get());
+      if (P->schedule >= 1) sht::build_as_schedule(P.get());
     } catch (...) {
       P->release();
       throw;

@@ -204,6 +204,7 @@ struct sht_plan {
   std::vector<int64_t> h_ta, h_tb, h_tc, h_kk;
   bool host_tasks = false;
   double flops = 0.0;
+  double sum_k = 0.0;     // contracted length summed over tasks
   double bytes_in = 0.0;  // A + B + C write
   double bytes_c = 0.0;   // C read (beta != 0)
   // A-stationary schedule (built on demand)

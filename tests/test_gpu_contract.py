@@ -39,9 +39,8 @@ EXACT = tuple(k for k in PRECS if k.startswith("fp64"))
 def make_plan(ctx, expr, dA, dB, dC, prec):
     plan = sht.Plan(ctx, expr, dA, dB, dC, PRECS[prec])
     sched = int(prec[-1]) if "_as" in prec else 0
-    if sched:
-        plan.set_schedule(sched)
-        assert plan.get_schedule() == sched
+    plan.set_schedule(sched)
+    assert plan.get_schedule() == sched
     return plan
 
 
