@@ -195,11 +195,26 @@ const char* sht_kclass_name(int kclass);
  * contracted length of at least 64 (not SHT_PREC_FP64_EXACT), else 0.     */
 sht_status sht_plan_set_schedule(sht_plan* p, int schedule);
 sht_status sht_plan_get_schedule(const sht_plan* p, int* schedule);
-/* Autotuner: time each applicable schedule (`reps` executions on `stream`)
- * and keep the fastest.  ms_out (4 entries, may be NULL) receives the time
- * of schedules 0-3 (-1 where not tried).
+/* Autotuner: time each applicable schedule and C-stationary tile variant
+ * (`reps` executions on `stream`) and keep the fastest.  ms_out (4 entries,
+ * may be NULL) receives the best time of schedules 0-3 (-1 where not tried).
  * C is overwritten by the trial executions.                                 */
 sht_status sht_plan_autotune(sht_plan* p, double alpha, double beta, int reps, void* stream, float* ms_out);
+/* The autotuner over schedules and C-stationary tile variants: every
+ * applicable (schedule, variant) pair is timed (`reps` executions after a
+ * warm-up) and the fastest kept.  *n_out receives the candidate count; the
+ * first `cap` candidates' schedule, variant and ms go to the arrays (each
+ * may be NULL).  C is overwritten by the trials.                            */
+sht_status sht_plan_autotune_ex(sht_plan* p, double alpha, double beta, int reps, void* stream, int cap, int* n_out,
+                                int* sched_out, int* variant_out, float* ms_out);
+/* C-stationary tile variant: 0 automatic (tile class by block width, 128-row
+ * tiles in large FP64/FP32 launches, XCD-aware order), 1 without 128-row
+ * tiles, 2 with 128-row tiles in any launch, 3 tiles by descending cost,
+ * 4 uniform 64 x 64 four-wave tiles for every block (128 x 32 for the fma
+ * chain): a fixed-tile reference.  Results do not depend on it.             */
+sht_status sht_plan_set_variant(sht_plan* p, int variant);
+sht_status sht_plan_get_variant(const sht_plan* p, int* variant);
+const char* sht_cs_variant_name(int variant);
 /* capture one execution into a hipGraph and replay it `replays` times       */
 sht_status sht_plan_graph_replay(sht_plan* p, double alpha, double beta, int replays, void* stream);
 /* task list (host copies): positions into A, B, C and the contracted-block

@@ -384,6 +384,26 @@ class Plan:
         arr = [np.ctypeslib.as_array(x, shape=(k,)).copy() if k else np.zeros(0, np.int64) for x in ptrs]
         return {"a": arr[0], "b": arr[1], "c": arr[2], "key": arr[3]}
 
+    def autotune_ex(self, alpha=1.0, beta=0.0, reps=2, stream=None):
+        """Time every (schedule, C-stationary tile variant) and keep the
+        fastest; returns [(schedule, variant name, ms), ...]."""
+        cap = 16
+        n = C.c_int()
+        sc, va = (C.c_int * cap)(), (C.c_int * cap)()
+        ms = (C.c_float * cap)()
+        call("sht_plan_autotune_ex", C.c_void_p(self.ptr), float(alpha), float(beta), int(reps), _sptr(stream), cap,
+             C.byref(n), sc, va, ms)
+        name = lambda v: lib().sht_cs_variant_name(v).decode()
+        return [(sc[i], name(va[i]), ms[i]) for i in range(min(n.value, cap))]
+
+    def set_variant(self, variant):
+        call("sht_plan_set_variant", C.c_void_p(self.ptr), int(variant))
+
+    def get_variant(self):
+        v = C.c_int()
+        call("sht_plan_get_variant", C.c_void_p(self.ptr), C.byref(v))
+        return v.value
+
     def build_ms(self):
         """Host wall time of the build steps (ms): task list + C pattern,
         C-stationary tiles, A-stationary schedule (-1 if not built)."""

@@ -49,6 +49,9 @@ SIGNATURES = {
     "sht_kernel_call_static": [_P, _P, _P, _P, _I64, _P],
     "sht_plan_build_ms": [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)],
     "sht_server_cancel": [_P, _I64],
+    "sht_plan_autotune_ex": [_P, _D, _D, _I, _P, _I, _PI, _PI, _PI, C.POINTER(C.c_float)],
+    "sht_plan_set_variant": [_P, _I],
+    "sht_plan_get_variant": [_P, _PI],
     "sht_server_counters": [_P, _PI64, _PI64, C.POINTER(_SZ)],
     "sht_kernel_info": [_P, C.c_char_p, _SZ],
     "sht_kernel_destroy": [_P],
@@ -115,6 +118,8 @@ def lib():
         L.sht_version.argtypes = []
         L.sht_kclass_name.restype = C.c_char_p
         L.sht_kclass_name.argtypes = [C.c_int]
+        L.sht_cs_variant_name.restype = C.c_char_p
+        L.sht_cs_variant_name.argtypes = [C.c_int]
         _lib = L
     return _lib
 

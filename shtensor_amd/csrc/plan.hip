@@ -465,6 +465,8 @@ KClass quad_rows(KClass k)
   }
 }
 
+void build_tiles(sht_plan* P);
+
 void build_schedule(sht_plan* P)
 {
   Arena& arena = P->ctx->arena;
@@ -501,6 +503,28 @@ void build_schedule(sht_plan* P)
   P->bytes_in = es * (static_cast<double>(P->A->nelem) + static_cast<double>(P->B->nelem) +
                       static_cast<double>(P->C->nelem));
   P->bytes_c = es * static_cast<double>(P->C->nelem);
+  P->h_cblk = std::move(cb);
+  P->h_work = std::move(w);
+  build_tiles(P);
+}
+
+// C-stationary tile lists of the plan's current variant (P->cs_variant):
+// 0 automatic (below), 1 without the 128-row tiles, 2 with 128-row tiles in
+// any launch (FP64 / FP32), 3 tiles by descending cost (no XCD dealing),
+// 4 uniform 64 x 64 four-wave tiles for every block (128 x 32 for the fma
+// chain): the fixed-tile reference the autotuner is measured against.
+void build_tiles(sht_plan* P)
+{
+  Arena& arena = P->ctx->arena;
+  const int64_t ncb = P->ncb;
+  const ContractMaps& m = P->maps;
+  const std::vector<CBlkRec>& cb = P->h_cblk;
+  const std::vector<int64_t>& w = P->h_work;
+  const int variant = P->cs_variant;
+  if (P->d_tiles) {
+    arena.release(P->d_tiles);
+    P->d_tiles = nullptr;
+  }
 
   // Tiles per kernel class.  When A is the larger operand (at least twice
   // B), an XCD-aware order: workgroup g runs on XCD g % 8 and every XCD has
@@ -542,11 +566,17 @@ void build_schedule(sht_plan* P)
   // its first MFMA) dominates, while small launches need every wave
   int64_t n64 = 0;
   for (int64_t c = 0; c < ncb; ++c) n64 += (cb[c].m + 63) / 64;
-  const bool quad = (P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32) && n64 > 64 * 1024;
+  const bool quad_ok = P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32;
+  const bool quad = quad_ok && (variant == 2 || (variant != 1 && variant != 4 && n64 > 64 * 1024));
+  const bool uniform = variant == 4;
   for (int64_t c = 0; c < ncb; ++c) {
     const int M = cb[c].m, N = cb[c].n;
     KClass k;
-    if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
+    if (uniform)
+      k = exact ? KClass::Exact32
+                : (P->prec == SHT_PREC_FP32 ? KClass::F32W
+                                            : (P->prec == SHT_PREC_BF16X3 ? KClass::Bf16W : KClass::Mfma64W));
+    else if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
     else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32N : (N <= 32 ? KClass::F32M : KClass::F32W);
     else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16N : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
     else if (N > 32) k = KClass::Mfma64W;
@@ -561,7 +591,8 @@ void build_schedule(sht_plan* P)
       // 32 columns, large launches), then 64; the last row tile of an MFMA class takes the
       // 32-row form when it can
       KClass kt = k;
-      if (!exact && M - m0 <= 32) kt = half_rows(k);
+      if (uniform) kt = k;
+      else if (!exact && M - m0 <= 32) kt = half_rows(k);
       else if (quad && M - m0 > 96) kt = quad_rows(k);
       for (int n0 = 0; n0 < N; n0 += cols) {
         const double cost =
@@ -577,7 +608,7 @@ void build_schedule(sht_plan* P)
   for (int k = 0; k < static_cast<int>(KClass::Count); ++k) {
     auto& v = per[k];
     if (v.empty()) continue;
-    if (!by_row && static_cast<int64_t>(v.size()) * kclass_waves(static_cast<KClass>(k)) < 32 * 1024) {
+    if (variant == 3 || (!by_row && static_cast<int64_t>(v.size()) * kclass_waves(static_cast<KClass>(k)) < 32 * 1024)) {
       std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
       P->launches.push_back({static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size())});
       for (const T& t : v) all.push_back(t.r);
@@ -1402,54 +1433,138 @@ sht_status sht_plan_get_schedule(const sht_plan* P, int* schedule)
   });
 }
 
-// Time every applicable schedule (reps executions each, after one warm-up,
-// on `stream`) and keep the fastest.  C is overwritten.
-sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, void* stream, float* ms_out)
+namespace {
+
+// Release a captured graph (its launches depend on the schedule and tiles).
+void drop_graph(sht_plan* P)
 {
-  return guard([&] {
-    SHT_REQUIRE(P && reps > 0, SHT_ERR_INVALID, "bad argument");
-    hipStream_t s = sht::as_stream(stream);
-    if (ms_out)
-      for (int c = 0; c < 4; ++c) ms_out[c] = -1.f;
-    std::vector<int> cand{0};
-    if (sht::as_supported(P)) {
-      cand.push_back(1);
-      // the fma-chain form has a single A-stationary kernel
-      if (P->prec != SHT_PREC_FP64_EXACT) cand.push_back(2);
-      if (sht::narrow_supported(P)) cand.push_back(3);
-    }
-    if (P->graph) {
-      SHT_HIP(hipGraphExecDestroy(P->graph));
-      SHT_HIP(hipGraphDestroy(P->graph_def));
-      P->graph = nullptr;
-      P->graph_def = nullptr;
-    }
-    hipEvent_t e0, e1;
-    SHT_HIP(hipEventCreate(&e0));
-    SHT_HIP(hipEventCreate(&e1));
-    int best = P->schedule;
-    float best_ms = 1e30f;
-    for (int c : cand) {
-      if (c >= 1) sht::build_as_schedule(P);
-      P->schedule = c;
+  if (P->graph) SHT_HIP(hipGraphExecDestroy(P->graph));
+  if (P->graph_def) SHT_HIP(hipGraphDestroy(P->graph_def));
+  P->graph = nullptr;
+  P->graph_def = nullptr;
+}
+
+// Every (schedule, C-stationary variant) the plan can run, timed with `reps`
+// executions after one warm-up; the fastest is kept.
+void autotune(sht_plan* P, double alpha, double beta, int reps, hipStream_t s, std::vector<int>& sched,
+              std::vector<int>& variant, std::vector<float>& ms)
+{
+  std::vector<std::pair<int, int>> cand;
+  const bool quad_ok = P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32;
+  for (int v = 0; v <= 4; ++v)
+    if (quad_ok || (v != 1 && v != 2)) cand.push_back({0, v});
+  if (sht::as_supported(P)) {
+    cand.push_back({1, 0});
+    // the fma-chain form has a single A-stationary kernel
+    if (P->prec != SHT_PREC_FP64_EXACT) cand.push_back({2, 0});
+    if (sht::narrow_supported(P)) cand.push_back({3, 0});
+  }
+  drop_graph(P);
+  hipEvent_t e0, e1;
+  SHT_HIP(hipEventCreate(&e0));
+  SHT_HIP(hipEventCreate(&e1));
+  int best_s = P->schedule, best_v = P->cs_variant;
+  float best_ms = 1e30f;
+  try {
+    for (const auto& c : cand) {
+      if (c.first >= 1) sht::build_as_schedule(P);
+      if (c.first == 0 && P->cs_variant != c.second) {
+        P->cs_variant = c.second;
+        sht::build_tiles(P);
+      }
+      P->schedule = c.first;
       sht::enqueue(P, alpha, beta, s);
       SHT_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < reps; ++r) sht::enqueue(P, alpha, beta, s);
       SHT_HIP(hipEventRecord(e1, s));
       SHT_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SHT_HIP(hipEventElapsedTime(&ms, e0, e1));
-      ms /= reps;
-      if (ms_out) ms_out[c] = ms;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = c;
+      float t = 0.f;
+      SHT_HIP(hipEventElapsedTime(&t, e0, e1));
+      t /= reps;
+      sched.push_back(c.first);
+      variant.push_back(c.second);
+      ms.push_back(t);
+      if (t < best_ms) {
+        best_ms = t;
+        best_s = c.first;
+        best_v = c.second;
       }
     }
+  } catch (...) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    P->schedule = best;
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (P->cs_variant != best_v) {
+    P->cs_variant = best_v;
+    sht::build_tiles(P);
+  }
+  P->schedule = best_s;
+}
+
+}  // namespace
+
+// Time every applicable schedule and C-stationary tile variant (reps
+// executions each, after one warm-up, on `stream`) and keep the fastest.
+// C is overwritten.
+sht_status sht_plan_autotune(sht_plan* P, double alpha, double beta, int reps, void* stream, float* ms_out)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && reps > 0, SHT_ERR_INVALID, "bad argument");
+    std::vector<int> sc, va;
+    std::vector<float> ms;
+    autotune(P, alpha, beta, reps, sht::as_stream(stream), sc, va, ms);
+    if (ms_out) {
+      for (int c = 0; c < 4; ++c) ms_out[c] = -1.f;
+      for (size_t i = 0; i < ms.size(); ++i)
+        if (ms_out[sc[i]] < 0.f || ms[i] < ms_out[sc[i]]) ms_out[sc[i]] = ms[i];
+    }
   });
+}
+
+sht_status sht_plan_autotune_ex(sht_plan* P, double alpha, double beta, int reps, void* stream, int cap, int* n_out,
+                                int* sched_out, int* variant_out, float* ms_out)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && reps > 0 && n_out, SHT_ERR_INVALID, "bad argument");
+    std::vector<int> sc, va;
+    std::vector<float> ms;
+    autotune(P, alpha, beta, reps, sht::as_stream(stream), sc, va, ms);
+    *n_out = static_cast<int>(ms.size());
+    for (int i = 0; i < cap && i < static_cast<int>(ms.size()); ++i) {
+      if (sched_out) sched_out[i] = sc[i];
+      if (variant_out) variant_out[i] = va[i];
+      if (ms_out) ms_out[i] = ms[i];
+    }
+  });
+}
+
+sht_status sht_plan_set_variant(sht_plan* P, int variant)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && variant >= 0 && variant <= 4, SHT_ERR_INVALID, "C-stationary variant must be 0..4");
+    if (variant != P->cs_variant) {
+      drop_graph(P);
+      P->cs_variant = variant;
+      sht::build_tiles(P);
+    }
+  });
+}
+
+sht_status sht_plan_get_variant(const sht_plan* P, int* variant)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && variant, SHT_ERR_INVALID, "null argument");
+    *variant = P->cs_variant;
+  });
+}
+
+const char* sht_cs_variant_name(int v)
+{
+  static const char* names[] = {"auto", "no_128row", "all_128row", "cost_order", "uniform_64x64"};
+  return v >= 0 && v <= 4 ? names[v] : "unknown";
 }
 
 sht_status sht_plan_tasks(sht_plan* P, const int64_t** ta, const int64_t** tb, const int64_t** tc,

@@ -194,6 +194,11 @@ struct sht_plan {
     int64_t first, count;
   };
   std::vector<Launch> launches;
+  // C-stationary tile variant (0 automatic .. 4 uniform 64 x 64) and the host
+  // copies of the C block records and per-block work it is built from
+  int cs_variant = 0;
+  std::vector<sht::CBlkRec> h_cblk;
+  std::vector<int64_t> h_work;
   // staging for non-canonical layouts
   bool permA = false, permB = false, permC = false;
   void* d_Ap = nullptr;

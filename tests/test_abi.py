@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_binding_covers_header():
-    bound = set(_lib.SIGNATURES) | {"sht_last_error", "sht_version", "sht_kclass_name"}
+    bound = set(_lib.SIGNATURES) | {"sht_last_error", "sht_version", "sht_kclass_name", "sht_cs_variant_name"}
     assert set(declared_symbols()) <= bound
 
 

@@ -401,3 +401,44 @@ def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
             assert np.array_equal(outs[sc], R.data), sc
         else:
             assert normwise(outs[sc], R.data) <= 1e-6, sc
+
+
+@pytest.mark.parametrize("prec", [sht.PREC_FP64, sht.PREC_FP64_EXACT, sht.PREC_FP32, sht.PREC_BF16X3])
+def test_tile_variants_and_autotune_ex(ctx, prec):
+    """Every C-stationary tile variant (automatic, without / with 128-row
+    tiles, cost order, uniform 64 x 64) gives the oracle's result (FP64 forms
+    bit for bit); the autotuner times every (schedule, variant) and the
+    variant it keeps gives the same result again."""
+    rng = np.random.default_rng(202)
+    expr = "ik,kj->ij"
+    bs_i = [5, 32, 33, 64, 97, 130, 161, 17]
+    bs_k = [7, 16, 40, 3]
+    bs_j = [3, 16, 17, 30, 9, 70, 40]
+    A = random_bst(rng, [bs_i, bs_k], 0.8)
+    B = random_bst(rng, [bs_k, bs_j], 0.8)
+    Cp = orc.BlockSparse([bs_i, bs_j])
+    _, _, _, R = oracle_result(expr, A, B, Cp)
+    dA, dB = to_device(ctx, A), to_device(ctx, B)
+    dC = sht.Tensor(ctx, Cp.block_sizes)
+    plan = sht.Plan(ctx, expr, dA, dB, dC, prec)
+    plan.set_schedule(0)
+
+    def check():
+        plan.execute(1.0, 0.0)
+        sht.call("sht_device_synchronize")
+        out = dC.download()
+        if prec in (sht.PREC_FP64, sht.PREC_FP64_EXACT):
+            assert np.array_equal(out, R.data)
+        else:
+            assert normwise(out, R.data) <= 1e-6
+
+    for v in range(5):
+        plan.set_variant(v)
+        assert plan.get_variant() == v
+        check()
+    cands = plan.autotune_ex(1.0, 0.0, reps=1)
+    names = {(s, v) for s, v, _ in cands}
+    assert (0, "uniform_64x64") in names and (0, "auto") in names
+    assert all(ms > 0 for _, _, ms in cands)
+    check()
+    plan.close()
