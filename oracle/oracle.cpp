@@ -11,6 +11,7 @@
 #include <immintrin.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cctype>
 #include <cmath>
@@ -626,9 +627,20 @@ int64_t orc_contract_f64(const char* expr, double alpha, const orc_bst* A, const
                          orc_bst* C, int mode, const int64_t* subset, int64_t nsubset, int nthreads,
                          char* err, int errlen)
 {
+  return orc_contract_f64_t(expr, alpha, A, B, beta, C, mode, subset, nsubset, nthreads, err, errlen, nullptr,
+                            nullptr);
+}
+
+int64_t orc_contract_f64_t(const char* expr, double alpha, const orc_bst* A, const orc_bst* B, double beta,
+                           orc_bst* C, int mode, const int64_t* subset, int64_t nsubset, int nthreads,
+                           char* err, int errlen, double* plan_s, double* compute_s)
+{
+  const auto tp0 = std::chrono::steady_clock::now();
   Plan P;
   std::string e;
   if (!make_plan(expr, A, B, C, P, e)) { set_err(err, errlen, e); return -1; }
+  const auto tp1 = std::chrono::steady_clock::now();
+  if (plan_s) *plan_s = std::chrono::duration<double>(tp1 - tp0).count();
   if ((int64_t)P.cids.size() != C->nnz) { set_err(err, errlen, "C pattern not reserved"); return -1; }
   for (int64_t i = 0; i < C->nnz; ++i)
     if (C->indices[i] != P.cids[i]) { set_err(err, errlen, "C pattern mismatch"); return -1; }
@@ -678,6 +690,7 @@ int64_t orc_contract_f64(const char* expr, double alpha, const orc_bst* A, const
     }
   };
   pool_run(0, nsubset, nthreads, body);
+  if (compute_s) *compute_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tp1).count();
   return ntask.load();
 }
 

@@ -95,6 +95,12 @@ int64_t orc_contract_f64(const char* expr, double alpha, const orc_bst* A, const
                          double beta, orc_bst* C, int mode, const int64_t* subset,
                          int64_t nsubset, int nthreads, char* err, int errlen);
 
+/* The same, reporting the wall time of building the task list and C
+ * pattern (plan_s) and of the block products (compute_s); either may be NULL. */
+int64_t orc_contract_f64_t(const char* expr, double alpha, const orc_bst* A, const orc_bst* B, double beta,
+                           orc_bst* C, int mode, const int64_t* subset, int64_t nsubset, int nthreads,
+                           char* err, int errlen, double* plan_s, double* compute_s);
+
 /* Block norms (FROBENIUS=0 accumulated in double, MAXABS=1) of every block,
  * src/Basics/BlockSpan.h:140-164 with the int accumulator fixed.           */
 void orc_block_norms_f64(const orc_bst* T, int norm, double* out);

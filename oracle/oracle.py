@@ -54,6 +54,10 @@ def lib():
         L.orc_contract_f64.argtypes = [C.c_char_p, C.c_double, C.POINTER(Bst), C.POINTER(Bst), C.c_double,
                                        C.POINTER(Bst), C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_char_p,
                                        C.c_int]
+        L.orc_contract_f64_t.restype = C.c_int64
+        L.orc_contract_f64_t.argtypes = [C.c_char_p, C.c_double, C.POINTER(Bst), C.POINTER(Bst), C.c_double,
+                                         C.POINTER(Bst), C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_char_p,
+                                         C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_block_norms_f64.restype = None
         L.orc_block_norms_f64.argtypes = [C.POINTER(Bst), C.c_int, F64P]
         L.orc_splitmix64.restype = C.c_uint64
@@ -213,8 +217,10 @@ def tasks(expr, A, B, Cpat):
     return t[:nt], kk[:nt], cid[:nc.value]
 
 
-def contract(expr, alpha, A, B, beta, Cten, mode=0, subset=None, nthreads=1):
-    """Compute C in place (pattern must be reserved).  Returns task count."""
+def contract(expr, alpha, A, B, beta, Cten, mode=0, subset=None, nthreads=1, timing=None):
+    """Compute C in place (pattern must be reserved).  Returns task count.
+    timing (a dict, optional) receives "plan_s" (task list + C pattern) and
+    "compute_s" (block products) wall times."""
     sa, sb, sc = A.struct(), B.struct(), Cten.struct()
     err = C.create_string_buffer(512)
     sub = None
@@ -222,10 +228,14 @@ def contract(expr, alpha, A, B, beta, Cten, mode=0, subset=None, nthreads=1):
     if subset is not None:
         subset = np.ascontiguousarray(subset, dtype=np.int64)
         sub, nsub = subset.ctypes.data, len(subset)
-    r = lib().orc_contract_f64(expr.encode(), alpha, C.byref(sa), C.byref(sb), beta, C.byref(sc), mode, sub, nsub,
-                               nthreads, err, 512)
+    ps, cs = C.c_double(0.0), C.c_double(0.0)
+    r = lib().orc_contract_f64_t(expr.encode(), alpha, C.byref(sa), C.byref(sb), beta, C.byref(sc), mode, sub, nsub,
+                                 nthreads, err, 512, C.byref(ps), C.byref(cs))
     if r < 0:
         raise OracleError(err.value.decode())
+    if timing is not None:
+        timing["plan_s"] = ps.value
+        timing["compute_s"] = cs.value
     return r
 
 
