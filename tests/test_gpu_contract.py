@@ -343,13 +343,13 @@ def test_large_launch_128_row_tiles(ctx, prec, names):
 def test_alike_operands_square_order(ctx, prec, name):
     """A and B of the same size, and more than 32 four-wave tiles per SIMD:
     the tiles are dealt to the XCD queues in squares of 16 x 16 C blocks
-    (9 216 C blocks of 32 x 32 in 96 x 96 block rows and columns, so the
-    squares at the edges are partial and the eight queues are padded); FP64
-    bit for bit equal to the oracle, FP32 and split-BF16 within their
-    tolerance."""
+    (about 10 000 C blocks of 32 x 32 in 100 x 100 block rows and columns:
+    100 = 6 x 16 + 4, so the last square of every row and column of squares
+    is partial, and the eight queues are padded); FP64 bit for bit equal to
+    the oracle, FP32 and split-BF16 within their tolerance."""
     rng = np.random.default_rng(83)
     expr = "ik,kj->ij"
-    bs = [32] * 96
+    bs = [32] * 100
     A = random_bst(rng, [bs, bs], 0.25)
     B = random_bst(rng, [bs, bs], 0.25)
     Cp = orc.BlockSparse([bs, bs])
@@ -442,3 +442,57 @@ def test_tile_variants_and_autotune_ex(ctx, prec):
     assert all(ms > 0 for _, _, ms in cands)
     check()
     plan.close()
+
+
+C1_FORMS = [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP64_EXACT, (0, 1)), (sht.PREC_FP32, (0, 1, 2, 3)),
+            (sht.PREC_BF16X3, (0, 1, 2))]
+
+
+@pytest.fixture(scope="module")
+def c1_recipe():
+    """Config 1's recipe at L = 180: ijkl,klm->ijm, uneven blocks drawn from
+    {5,7,9,11,13,15,21,28}, p = 0.1, values decaying by exp(-0.3|b0-b1|),
+    seed 1001 -- generated on the device and by the oracle from the same
+    counters; the oracle's contraction of it."""
+    seed, L, choices = 1001, 180, [5, 7, 9, 11, 13, 15, 21, 28]
+    bs = [orc.gen_block_sizes(seed, ax, L, choices).tolist() for ax in range(5)]
+    bsA, bsB, bsC = [bs[0], bs[1], bs[2], bs[3]], [bs[2], bs[3], bs[4]], [bs[0], bs[1], bs[4]]
+    A = orc.BlockSparse(bsA).reserve_ids(orc.gen_presence(seed, 0, [len(b) for b in bsA], 0.1))
+    B = orc.BlockSparse(bsB).reserve_ids(orc.gen_presence(seed, 1, [len(b) for b in bsB], 0.1))
+    orc.gen_values(seed, 0, A, 0.3)
+    orc.gen_values(seed, 1, B, 0.3)
+    _, _, _, R = oracle_result("ijkl,klm->ijm", A, B, orc.BlockSparse(bsC))
+    return seed, bsA, bsB, bsC, R
+
+
+@pytest.mark.parametrize("prec,scheds", C1_FORMS)
+def test_config1_recipe_every_schedule(ctx, c1_recipe, prec, scheds):
+    """Every precision form through every schedule on config 1's block
+    recipe: FP64 forms bit for bit, FP32 and split-BF16 within 1e-6
+    normwise and max-blockwise."""
+    seed, bsA, bsB, bsC, R = c1_recipe
+    dA = sht.Tensor(ctx, bsA)
+    dA.reserve_random(seed, 0, 0.1)
+    dA.fill_random(seed, 0, 0.3)
+    dB = sht.Tensor(ctx, bsB)
+    dB.reserve_random(seed, 1, 0.1)
+    dB.fill_random(seed, 1, 0.3)
+    ends = np.r_[R.offsets[1:], len(R.data)]
+    for sc in scheds:
+        print("config 1 recipe", prec, "schedule", sc, flush=True)
+        dC = sht.Tensor(ctx, bsC)
+        plan = sht.Plan(ctx, "ijkl,klm->ijm", dA, dB, dC, prec)
+        plan.set_schedule(sc)
+        plan.execute(1.0, 0.0)
+        sht.call("sht_device_synchronize")
+        assert np.array_equal(dC.get_indices(), R.indices)
+        out = dC.download()
+        if prec in (sht.PREC_FP64, sht.PREC_FP64_EXACT):
+            assert np.array_equal(out, R.data), sc
+        else:
+            assert normwise(out, R.data) <= 1e-6, sc
+            blk = max(np.linalg.norm(out[a:b] - R.data[a:b]) / np.linalg.norm(R.data[a:b])
+                      for a, b in zip(R.offsets, ends) if np.linalg.norm(R.data[a:b]) > 0)
+            assert blk <= 1e-6, (sc, blk)
+        plan.close()
+        dC.close()

@@ -196,3 +196,23 @@ def test_diag_mfma_rate():
     than clock slack)."""
     r = sht.diag_mfma_f64_rate()
     assert 5.0 < r < 90.0, r
+
+
+@pytest.mark.parametrize("expr,sa,sb,sc", [CASES[0], CASES[1], CASES[2], CASES[9], CASES[14]])
+def test_lapack_typed_kernel_vs_lapack_oracle(expr, sa, sb, sc):
+    """A kernel created with KernelType::LAPACK (one HIP backend: the same
+    kernels) against the oracle's reshape + netlib dgemm restatement of the
+    reference's LAPACK variant (src/Kernel/KernelImpl.cpp:142-208): the two
+    orders of summation agree to 1e-13 normwise."""
+    rng = np.random.default_rng(zlib.crc32(expr.encode()))
+    nb = 6
+    a = rng.uniform(-1, 1, nb * int(np.prod(sa)))
+    b = rng.uniform(-1, 1, nb * int(np.prod(sb)))
+    c = rng.uniform(-1, 1, nb * int(np.prod(sc)))
+    want = c.copy()
+    orc.kernel_lapack(expr, sa, sb, sc, 0.75, 1.5, a, b, want, nb)
+    k = sht.Kernel(expr, sa, sb, sc, 0.75, 1.5, sht.LAPACK, np.float64)
+    got = c.copy()
+    k.call(a, b, got, nb)
+    k.close()
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)

@@ -130,6 +130,40 @@ def test_contract_float_chain_vs_longdouble(expr, ea, eb, ec):
     assert np.allclose(Cp.data, C1.data, rtol=1e-13, atol=1e-14)
 
 
+@pytest.mark.parametrize("expr,ea,eb,ec", CONTRACT_CASES)
+def test_contract_chain_is_kernel_chain_per_task(expr, ea, eb, ec):
+    """Float data: the contraction's cross-task rule is exactly the block
+    Kernel's chain (kernel_xmm) applied task by task in key order -- the
+    first task with the contraction's beta, every later one with beta = 1
+    (acc = 1*C is exact, so the fma chain simply continues).  Pins the order
+    of summation across blocks, which integer data and long-double
+    comparisons cannot see."""
+    rng = np.random.default_rng(17)
+    A, B, Cp = make_case(rng, ea, eb, ec, p=0.7)
+    t, _, cids = orc.tasks(expr, A, B, Cp)
+    C = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+    c0 = rng.uniform(-1, 1, len(C.data))
+    C.data[:] = c0
+    W = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+    W.data[:] = c0
+    alpha, beta = 0.7, -1.3
+    orc.contract(expr, alpha, A, B, beta, C, mode=0)
+    ends_c = np.r_[W.offsets[1:], len(W.data)]
+    seen = set()
+    for pa, pb, pc in t:  # tasks sorted by (C block, contracted key)
+        da = A.dims(A.indices[pa])
+        db = B.dims(B.indices[pb])
+        dc = W.dims(W.indices[pc])
+        a = A.block(pa).ravel(order="F")
+        b = B.block(pb).ravel(order="F")
+        cblk = W.data[W.offsets[pc]:ends_c[pc]].copy()
+        orc.kernel_xmm(expr, da, db, dc, alpha, 1.0 if pc in seen else beta, a, b, cblk, 1)
+        W.data[W.offsets[pc]:ends_c[pc]] = cblk
+        seen.add(pc)
+    assert len(seen) == len(cids)  # (every C block of the product pattern has a task)
+    assert np.array_equal(C.data.view(np.uint64), W.data.view(np.uint64))
+
+
 def test_kept_pattern_drops_outside_tasks():
     rng = np.random.default_rng(21)
     A, B, Cp = make_case(rng, "ik", "kj", "ij", p=0.8, integer=True)
