@@ -1018,8 +1018,13 @@ void build_as_schedule_impl(sht_plan* P)
   // XCD-aware order.  Workgroup g runs on XCD g % 8 (round-robin dispatch),
   // and every XCD has its own L2: all units of a block row go to one XCD, so
   // the row's panels are shared in that L2.  Rows are dealt to the XCD with
-  // the least accumulated work (longest first), then the eight queues are
-  // interleaved; short queues are padded with empty units.
+  // the least accumulated work, then the eight queues are interleaved; short
+  // queues are padded with empty units.  Rows are taken in descending order
+  // of their sweep-list cost, which is the running time of each of the row's
+  // workgroups (every one walks the whole list), not of the row's total
+  // work: longest-processing-time-first keeps the last round of workgroups
+  // short (C1: 52.8 -> 49.8 ms FP64, 38.4 -> 36.6 ms FP32 against ordering by
+  // total row work).
   constexpr int kXcd = 8;
   // Units of one block row are grouped per_wg to a workgroup (padded with
   // ghost units, so a workgroup never mixes rows): the waves of a workgroup
@@ -1028,9 +1033,7 @@ void build_as_schedule_impl(sht_plan* P)
   auto build = [&](const std::vector<Row>& rows, int step, int per_wg) {
     std::vector<int64_t> ro(rows.size());
     std::iota(ro.begin(), ro.end(), 0);
-    std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) {
-      return rows[x].work * rows[x].rows > rows[y].work * rows[y].rows;
-    });
+    std::stable_sort(ro.begin(), ro.end(), [&](int64_t x, int64_t y) { return rows[x].work > rows[y].work; });
     std::vector<std::vector<AsUnit>> q(kXcd);
     std::vector<double> load(kXcd, 0.0);
     for (int64_t r : ro) {
