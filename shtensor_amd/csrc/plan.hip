@@ -610,7 +610,10 @@ void build_tiles(sht_plan* P)
     if (v.empty()) continue;
     if (variant == 3 || (!by_row && static_cast<int64_t>(v.size()) * kclass_waves(static_cast<KClass>(k)) < 32 * 1024)) {
       std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
-      P->launches.push_back({static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size())});
+      double ck = 0.0;
+      for (const T& t : v) ck += t.cost;
+      P->launches.push_back(
+          {static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size()), ck});
       for (const T& t : v) all.push_back(t.r);
       continue;
     }
@@ -643,7 +646,9 @@ void build_tiles(sht_plan* P)
     for (size_t slot = 0; slot < longest; ++slot)
       for (int x = 0; x < kXcd; ++x) all.push_back(slot < q[x].size() ? q[x][slot] : TileRec{0, 0, 0, 1});
     while (!all.empty() && static_cast<int64_t>(all.size()) > first && all.back().pad) all.pop_back();
-    P->launches.push_back({static_cast<KClass>(k), first, static_cast<int64_t>(all.size()) - first});
+    double ck = 0.0;
+    for (const RowSpan& r : rows) ck += r.cost;
+    P->launches.push_back({static_cast<KClass>(k), first, static_cast<int64_t>(all.size()) - first, ck});
   }
   if (!all.empty()) {
     P->d_tiles = static_cast<TileRec*>(arena.allocate(all.size() * sizeof(TileRec)));
@@ -1212,6 +1217,29 @@ void enqueue_stage_out(sht_plan* P, hipStream_t s)
   if (P->permC) permute(P->C, P->d_Cp, P->C->d_data, P->orderC, 1, s);
 }
 
+// Streams for the C-stationary classes of one execution: one per class, at
+// most four (the caller's and three plan-owned ones; SHT_CS_STREAMS=1..4
+// overrides the maximum).
+int cs_streams(const sht_plan* P)
+{
+  static const int cap = [] {
+    const char* e = std::getenv("SHT_CS_STREAMS");
+    const int v = e ? std::atoi(e) : 1 + sht_plan::kSideStreams;
+    return std::max(1, std::min(v, 1 + sht_plan::kSideStreams));
+  }();
+  return static_cast<int>(std::min<size_t>(P->launches.size(), static_cast<size_t>(cap)));
+}
+
+// Non-blocking streams of default priority, created on first use.
+void ensure_side_streams(sht_plan* P, int n)
+{
+  if (!P->fork_ev) SHT_HIP(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming));
+  for (int k = 0; k < n; ++k) {
+    if (!P->side[k]) SHT_HIP(hipStreamCreateWithFlags(&P->side[k], hipStreamNonBlocking));
+    if (!P->join_ev[k]) SHT_HIP(hipEventCreateWithFlags(&P->join_ev[k], hipEventDisableTiming));
+  }
+}
+
 void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<hipEvent_t>* ev)
 {
   const bool az = zero_bytes(alpha), bz = zero_bytes(beta);
@@ -1258,14 +1286,43 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   a.alpha_zero = az ? 1 : 0;
   a.beta_zero = bz ? 1 : 0;
   a.alpha_one = alpha == 1.0 ? 1 : 0;
+  // Tile classes are independent (every C element belongs to one tile of one
+  // class), so with several classes they run concurrently: the caller's
+  // stream forks to up to three plan-owned streams and joins them again, so
+  // the execution stays ordered on s (and is captured whole into a graph).
+  // Classes go to the least-loaded stream, largest first.  Timed executions
+  // keep one stream so that each class's events bracket it alone.
+  const int nstr = ev ? 1 : cs_streams(P);
+  std::vector<int> on(P->launches.size(), 0);
+  if (nstr > 1) {
+    ensure_side_streams(P, nstr - 1);
+    std::vector<size_t> ord(P->launches.size());
+    std::iota(ord.begin(), ord.end(), size_t{0});
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](size_t x, size_t y) { return P->launches[x].cost > P->launches[y].cost; });
+    std::vector<double> load(nstr, 0.0);
+    for (size_t i : ord) {
+      const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      on[i] = k;
+      load[k] += P->launches[i].cost;
+    }
+    SHT_HIP(hipEventRecord(P->fork_ev, s));
+    for (int k = 0; k + 1 < nstr; ++k) SHT_HIP(hipStreamWaitEvent(P->side[k], P->fork_ev, 0));
+  }
   for (size_t i = 0; i < P->launches.size(); ++i) {
     const auto& L = P->launches[i];
     a.tiles = P->d_tiles + L.first;
     a.ntiles = L.count;
-    if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i], s));
-    launch_exec(L.k, a, s);
-    if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i + 1], s));
+    hipStream_t si = on[i] == 0 ? s : P->side[on[i] - 1];
+    if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i], si));
+    launch_exec(L.k, a, si);
+    if (ev) SHT_HIP(hipEventRecord((*ev)[2 * i + 1], si));
   }
+  if (nstr > 1)
+    for (int k = 0; k + 1 < nstr; ++k) {
+      SHT_HIP(hipEventRecord(P->join_ev[k], P->side[k]));
+      SHT_HIP(hipStreamWaitEvent(s, P->join_ev[k], 0));
+    }
   enqueue_stage_out(P, s);
 }
 

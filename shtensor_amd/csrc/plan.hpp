@@ -192,8 +192,15 @@ struct sht_plan {
   struct Launch {
     sht::KClass k;
     int64_t first, count;
+    double cost;  // summed tile cost estimates (stream dealing)
   };
   std::vector<Launch> launches;
+  // plan-owned streams for concurrent C-stationary classes, with the events
+  // that fork them from and join them to the caller's stream
+  static constexpr int kSideStreams = 3;
+  hipStream_t side[kSideStreams] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kSideStreams] = {nullptr, nullptr, nullptr};
   // C-stationary tile variant (0 automatic .. 4 uniform 64 x 64) and the host
   // copies of the C block records and per-block work it is built from
   int cs_variant = 0;
@@ -249,6 +256,11 @@ struct sht_plan {
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);
+    for (int i = 0; i < kSideStreams; ++i) {
+      if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+      if (side[i]) (void)hipStreamDestroy(side[i]);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
   }
 };
 
