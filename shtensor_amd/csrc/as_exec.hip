@@ -228,13 +228,14 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
   // steps past the contracted length (the loop runs whole 16-k iterations)
   // issue no MFMAs: every product they hold is zero
   const int kq = (K + 3) >> 2;
+  const double apad = pad_a(p.alpha, A1);
   auto mma = [&](int st, const double(&araw)[R], const TB(&b)[NS]) {
     if (st >= kq) return;
     const bool kin = 4 * st + lk < K;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if constexpr (F == 0) {
-        const double y = kin ? araw[t] : 0.0;
+        const double y = kin ? araw[t] : apad;
         const double a = A1 ? y : y * p.alpha;
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a, acc[t][q], 0, 0, 0);
@@ -415,17 +416,18 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + rl2);
   const uint32_t n4m = static_cast<uint32_t>(4 * nxs.m);
   TB b[NS];
-  auto mma = [&](int st, const double(&araw)[R]) {
-    const bool kin = 4 * st + lk < K;
+  // one step's MFMAs (the A values already masked where the step passes K)
+  const int kfull = K >> 2;  // steps whose four contracted indices are < K
+  const double apad = pad_a(p.alpha, A1);
+  auto mma_full = [&](const double(&araw)[R]) {
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if constexpr (F == 0) {
-        const double y = kin ? araw[t] : 0.0;
-        const double a = A1 ? y : y * p.alpha;
+        const double a = A1 ? araw[t] : araw[t] * p.alpha;
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[q], a, acc[t][q], 0, 0, 0);
       } else {
-        const float a = kin ? static_cast<float>(araw[t]) : 0.f;
+        const float a = static_cast<float>(araw[t]);
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[q], a, acc[t][q], 0, 0, 0);
       }
@@ -453,7 +455,23 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
     if (!ghost && st + j < kq) {
 #pragma unroll
       for (int q = 0; q < NS; ++q) b[q] = stage[j][q][lane];
-      mma(st + j, ar[j]);
+      // Only the last step of a contracted length that is not a multiple of
+      // 4 needs the k < K mask: a wave-uniform branch masks the A values of
+      // that step (the empty asm statement keeps the branch from being
+      // turned back into per-step selects), and every step runs the same
+      // MFMA sequence.
+      double am[R];
+#pragma unroll
+      for (int t = 0; t < R; ++t) am[t] = ar[j][t];
+      if (st + j >= kfull) {
+        const bool kin = 4 * (st + j) + lk < K;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+          asm volatile("" : "+v"(am[t]));
+          am[t] = kin ? am[t] : (F == 0 ? apad : 0.0);
+        }
+      }
+      mma_full(am);
     }
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (j % 2 == 1) {  // steps s+1, s+2 into the stages of steps s-3, s-2

@@ -154,6 +154,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
         const int j = tile.n0 + 16 * ct + 4 * v + lk;
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
+  const double apad = pad_a(p.alpha, p.alpha_one != 0);
   if (!p.alpha_zero) {
     for (int t = cb.t0; t < cb.t1; ++t) {
       const TaskRec tr = p.tasks[t];
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
           const int i = tile.m0 + 16 * rt + li;
-          a[rt] = (kin && i < M) ? Ab[i + static_cast<int64_t>(kk) * M] : 0.0;
+          a[rt] = (kin && i < M) ? Ab[i + static_cast<int64_t>(kk) * M] : apad;
           if (!p.alpha_one) a[rt] *= p.alpha;
         }
 #pragma unroll
@@ -279,6 +280,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
     bool lastl;    // the loaded chunk is the last of its task (uniform)
     const double alpha = p.alpha;
     const bool a1 = p.alpha_one != 0;
+    const double apad = pad_a(alpha, a1);
     v4f part[TRt][TCt];
 #pragma unroll
     for (int rt = 0; rt < TRt; ++rt)
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 #define SHT_W_STORE(stg)                                                                   \
   {                                                                                        \
     _Pragma("unroll") for (int e = 0; e < EA; ++e) {                                       \
-      const double x = ka ? ra[e] : 0.0;                                                   \
+      const double x = ka ? ra[e] : apad;                                                  \
       if constexpr (F == 0) sA[stg][kA][rA + e] = a1 ? x : x * alpha;                      \
       else sA[stg][kA][rA + e] = static_cast<float>(x);                                    \
     }                                                                                      \

@@ -8,6 +8,19 @@
 
 namespace sht {
 
+#ifdef __HIPCC__
+// The A value of a padded contracted position (k >= K in the last 4-k step
+// of an MFMA): a zero whose product with alpha, and then with the zero
+// padded B value, is -0.  x + (-0) = x for every x, -0 included, so the
+// padded products of an MFMA leave each fma chain's value and sign exactly
+// where the reference's chain over k < K ends (a +0 product would turn a
+// chain that ends at -0 into +0).
+__device__ inline double pad_a(double alpha, bool alpha_one)
+{
+  return alpha_one ? -0.0 : __builtin_copysign(0.0, -alpha);
+}
+#endif
+
 // One C-stationary tile: rows [m0, m0+rows) and columns [n0, n0+cols) of
 // C block `cb`; the kernel class fixes rows/cols.  pad = 1: a padding tile
 // (no work) that keeps the XCD-interleaved tile order aligned.

@@ -174,6 +174,53 @@ def test_alpha_zero_scales_only(ctx):
     assert np.array_equal(dC.download(), 3.0 * Ck.data)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp64_as1", "fp64_as2", "fp64_as3", "fp64_exact", "fp64_exact_as1"])
+@pytest.mark.parametrize("case", ["neg_zero", "neg_zero_alpha", "inf_last_column"])
+def test_padded_k_keeps_signed_zero_and_inf(ctx, prec, case):
+    """Contracted lengths that are not multiples of 4 leave padded positions
+    in the last step of every MFMA (the A stream re-reads column K-1 there).
+    Their products must not change any chain: a C element whose reference
+    chain ends at -0 stays -0, and an inf in column K-1 stays inf rather
+    than becoming NaN (inf * 0).  Compared bit for bit with the oracle."""
+    rng = np.random.default_rng(91)
+    expr = "ik,kj->ij"
+    bs_i, bs_k, bs_j = [16, 40, 7, 33], [5, 13, 6, 21], [16, 20, 33, 3]
+    A = random_bst(rng, [bs_i, bs_k], 0.9)
+    B = random_bst(rng, [bs_k, bs_j], 0.9)
+    Cp = orc.BlockSparse([bs_i, bs_j])
+    _, _, cids = orc.tasks(expr, A, B, Cp)
+    Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+    alpha, beta = 1.0, 1.0
+    if case in ("neg_zero", "neg_zero_alpha"):
+        # every product is -0 and C starts at -0: the chains end at -0
+        B.data[:] = rng.uniform(-1.0, -0.5, len(B.data))
+        A.data[:] = 0.0 if case == "neg_zero" else -0.0
+        alpha = 1.0 if case == "neg_zero" else -0.5
+        c0 = np.full(len(Ck.data), -0.0)
+    else:
+        # the last contracted column of every A block is +inf, B > 0
+        B.data[:] = rng.uniform(0.5, 1.0, len(B.data))
+        for p in range(len(A.indices)):
+            ik = np.unravel_index(int(A.indices[p]), (len(bs_i), len(bs_k)), order="F")
+            m, k = bs_i[ik[0]], bs_k[ik[1]]
+            A.data[A.offsets[p] + (k - 1) * m:A.offsets[p] + k * m] = np.inf
+        beta = 0.5
+        c0 = rng.uniform(-1, 1, len(Ck.data))
+    _, _, _, R = oracle_result(expr, A, B, Ck, alpha=alpha, beta=beta, c0=c0)
+    if case != "inf_last_column":
+        assert np.signbit(R.data).all() and not R.data.any()
+    else:
+        assert np.isinf(R.data).any() and not np.isnan(R.data).any()
+    Ck.data[:] = c0
+    dA, dB, dC = to_device(ctx, A), to_device(ctx, B), to_device(ctx, Ck)
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
+    plan.execute(alpha, beta)
+    sht.call("sht_device_synchronize")
+    plan.close()
+    out = dC.download()
+    assert np.array_equal(out.view(np.int64), R.data.view(np.int64))
+
+
 def test_empty_operand(ctx):
     A = orc.BlockSparse([[2, 3], [4]])
     A.reserve_ids([])
