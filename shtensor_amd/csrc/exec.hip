@@ -250,8 +250,16 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   const double* A = static_cast<const double*>(p.A);
   const double* B = static_cast<const double*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ TL sA[2][16][TR];  // [stage][k][row of the tile]
-  __shared__ TL sB[2][16][TC];  // [stage][k][column of the tile]
+  // Row padding of the LDS stages against bank conflicts of the loaders'
+  // stores and the waves' fragment reads, per tile shape (measured on C3
+  // b=32, 32 x 32 tiles: FP64 0.315-0.33 -> 0.26-0.31 ms, FP32 0.221 -> 0.194
+  // ms on one box and unchanged on another; C0's 64 x 16 FP64 tiles 0.0426 ->
+  // 0.0407 ms; no gain measured for the other shapes, which keep none).
+  constexpr bool sq32 = TR == 32 && TC == 32;
+  constexpr int PA = F == 0 ? (sq32 ? 1 : 0) : (sq32 ? 16 : 0);
+  constexpr int PB = F == 0 ? (sq32 ? 1 : (TC == 16 ? 1 : 0)) : (sq32 ? 18 : 0);
+  __shared__ TL sA[2][16][TR + PA];  // [stage][k][row of the tile]
+  __shared__ TL sB[2][16][TC + PB];  // [stage][k][column of the tile]
   // column of accumulator element v (the f64 and f32 MFMA results differ)
   auto colof = [&](int ct, int v) { return tile.n0 + wc + 16 * ct + (F == 0 ? 4 * v + lk : 4 * lk + v); };
   v4d acc[TRt][TCt];
