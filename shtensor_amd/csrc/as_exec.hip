@@ -408,12 +408,18 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // at step nstepsU the stream moves on to the next sweep's A block.  Ghost
   // waves use K = 1 (one fixed line).
   const double* abase = static_cast<const double*>(p.A) + sw.a_off;
-  uint32_t arun = static_cast<uint32_t>((4 * (AD - 1) + lk) * M + rl2);
-  uint32_t alast = static_cast<uint32_t>((ghost ? 0 : K - 1) * M + rl2);
+  // (a lane whose two rows are both past the block's last row re-reads rows
+  // 0 and 1 instead of running on into the next columns: its products only
+  // reach padding rows of the C tiles; a lane holding the last real row and
+  // one padding row keeps its pair, whose second row is the next column's
+  // first)
+  const int ra2 = rl2 < M ? rl2 : 0;
+  uint32_t arun = static_cast<uint32_t>((4 * (AD - 1) + lk) * M + ra2);
+  uint32_t alast = static_cast<uint32_t>((ghost ? 0 : K - 1) * M + ra2);
   uint32_t a4m = static_cast<uint32_t>(4 * M);
   const double* nbase = static_cast<const double*>(p.A) + nxs.a_off;
-  const uint32_t nrun = static_cast<uint32_t>(lk * nxs.m + rl2);
-  const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + rl2);
+  const uint32_t nrun = static_cast<uint32_t>(lk * nxs.m + ra2);
+  const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + ra2);
   const uint32_t n4m = static_cast<uint32_t>(4 * nxs.m);
   TB b[NS];
   // one step's MFMAs (the A values already masked where the step passes K)
@@ -576,7 +582,8 @@ __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
     const int kmax = ghost ? 0 : cur.k - 1;
 #pragma unroll
     for (int i = 0; i < AD - 1; ++i)
-      load_a_pair(a, static_cast<uint32_t>(min(4 * i + lk, kmax) * cur.m + r0 + 2 * li), ar[i]);
+      load_a_pair(a, static_cast<uint32_t>(min(4 * i + lk, kmax) * cur.m + (r0 + 2 * li < cur.m ? r0 + 2 * li : 0)),
+                  ar[i]);
   }
   for (int j = 0; j < nsw; ++j) {
     cbs[lane] = cc0;
@@ -813,6 +820,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
   // lane li holds rows r0 + 2li (tile 0) and r0 + 2li + 1 (tile 1): one
   // 16-byte load covers both tiles' rows of an A or C column
   const int rl2 = r0 + 2 * li;
+  const int ra2 = rl2 < M ? rl2 : 0;  // A rows (a pair of padding rows re-reads rows 0, 1)
   const int nsteps = (K + kBfK - 1) / kBfK;
   const int nst2 = (nsteps + 1) & ~1;  // the loop runs whole pairs of steps
   v4f acc[R][NS];
@@ -844,7 +852,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int kk = kBfK * min(st, nsteps - 1) + 8 * lk + e;
-      const uint32_t off = ghost ? static_cast<uint32_t>(rl2) : static_cast<uint32_t>(min(kk, K - 1) * M + rl2);
+      const uint32_t off = ghost ? static_cast<uint32_t>(ra2) : static_cast<uint32_t>(min(kk, K - 1) * M + ra2);
       const v2d8 v = *reinterpret_cast<const v2d8*>(A + off);
       x[0][e] = v.x;
       x[1][e] = v.y;
