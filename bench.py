@@ -753,6 +753,18 @@ def main():
         roof["mfma_loop_tflops"] = round(loop, 2)
         roof["frac_of_mfma_loop"] = round(roof["achieved"] / loop, 4)
         log("FP64 MFMA loop on this device: %.2f TFLOP/s" % loop)
+    if dom and dom[0] in ("as_mfma64_g", "as_mfma32_g"):
+        # the grouped A-stationary kernel's own step shape with its operands
+        # in cache (sweeps of 4 and 8 slots; config 1's sweeps average about
+        # 4.2 slots, weighted by steps): the ceiling of that loop, apart
+        # from memory misses, sweep boundaries, ghost waves and the tail
+        shape = {}
+        for nm, pr in (("fp64", sht.PREC_FP64), ("fp32", sht.PREC_FP32)):
+            shape[nm] = {str(ns): round(sht.diag_step_rate(pr, ns, stream), 2) for ns in (4, 8)}
+        roof["step_shape_tflops"] = shape
+        key = "fp64" if dom[0] == "as_mfma64_g" else "fp32"
+        roof["frac_of_step_shape_4"] = round(roof["achieved"] / shape[key]["4"], 4)
+        log("grouped-kernel step shape (operands in cache): %s" % shape)
     fp64_gf = None
     if "fp64" in results:
         fp64_gf = results["fp64"]["stats"]["flops"] / float(np.median(results["fp64"]["times"])) / 1e9

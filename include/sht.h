@@ -84,6 +84,13 @@ sht_status sht_stream_synchronize(void* stream);
  * v_mfma_f64_16x16x4f64 (two waves per SIMD), run on `stream`: the matrix-core
  * rate this device sustains, for comparison with the contraction kernels.   */
 sht_status sht_diag_mfma_f64_rate(void* stream, double* tflops);
+/* Rate (TFLOP/s, 2048 flop per MFMA) of the grouped A-stationary kernel's
+ * step shape with every operand in cache: per 4-k step one 16-byte A load,
+ * `slots` panel fragments from an LDS ring and 2 x slots MFMAs, a workgroup
+ * barrier every two steps, two 4-wave workgroups per CU (prec SHT_PREC_FP64
+ * or SHT_PREC_FP32, slots 1..8): the ceiling of that loop without memory
+ * misses, sweep boundaries, ghost waves or a tail.                          */
+sht_status sht_diag_step_rate(sht_precision prec, int slots, void* stream, double* tflops);
 
 /* ---- block-product kernel (replaces Kernel<T>, src/Kernel/Kernel.h:37-93,
  *      KernelImpl.cpp:9-73 ctor, 210-770 XMM JIT, 142-208 LAPACK) ----------

@@ -487,6 +487,15 @@ def diag_mfma_f64_rate(stream=None):
     return v.value
 
 
+def diag_step_rate(precision, slots, stream=None):
+    """TFLOP/s of the grouped A-stationary kernel's step shape (A load, LDS
+    panel fragments, 2 x slots MFMAs, a barrier every two steps) with every
+    operand in cache: that loop's ceiling (PREC_FP64 or PREC_FP32)."""
+    v = C.c_double()
+    call("sht_diag_step_rate", int(precision), int(slots), _sptr(stream), C.byref(v))
+    return v.value
+
+
 def contract(ctx, expr, alpha, A, B, beta, Cten, precision=PREC_FP64, stream=None):
     """C = beta*C + alpha*contract(expr, A, B) on the GPU (synchronous)."""
     call("sht_contract", C.c_void_p(ctx.ptr), expr.encode(), float(alpha), C.c_void_p(A.ptr), C.c_void_p(B.ptr),

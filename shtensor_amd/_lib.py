@@ -44,6 +44,7 @@ SIGNATURES = {
     "sht_stream_destroy": [_P],
     "sht_stream_synchronize": [_P],
     "sht_diag_mfma_f64_rate": [_P, C.POINTER(C.c_double)],
+    "sht_diag_step_rate": [C.c_int, C.c_int, _P, C.POINTER(C.c_double)],
     "sht_kernel_create": [C.c_char_p, _PI, _I, _PI, _I, _PI, _I, _D, _D, _I, _I, _PP],
     "sht_kernel_call": [_P, _P, _P, _P, _I64, _P],
     "sht_kernel_call_static": [_P, _P, _P, _P, _I64, _P],

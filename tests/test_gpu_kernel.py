@@ -198,6 +198,20 @@ def test_diag_mfma_rate():
     assert 5.0 < r < 90.0, r
 
 
+def test_diag_step_rate():
+    """The grouped kernel's step-shape probe: FP64 within the FP64 peak,
+    FP32 within the FP32 peak, wider sweeps no slower than narrow ones by
+    much, and out-of-range arguments rejected."""
+    r64 = {ns: sht.diag_step_rate(sht.PREC_FP64, ns) for ns in (1, 4, 8)}
+    r32 = {ns: sht.diag_step_rate(sht.PREC_FP32, ns) for ns in (1, 4, 8)}
+    assert all(2.0 < v < 90.0 for v in r64.values()), r64
+    assert all(2.0 < v < 180.0 for v in r32.values()), r32
+    assert r64[8] > 0.8 * r64[1] and r32[8] > 0.8 * r32[1], (r64, r32)
+    for bad in ((sht.PREC_FP64, 0), (sht.PREC_FP64, 9), (sht.PREC_BF16X3, 4)):
+        with pytest.raises(sht.ShtError):
+            sht.diag_step_rate(*bad)
+
+
 @pytest.mark.parametrize("expr,sa,sb,sc", [CASES[0], CASES[1], CASES[2], CASES[9], CASES[14]])
 def test_lapack_typed_kernel_vs_lapack_oracle(expr, sa, sb, sc):
     """A kernel created with KernelType::LAPACK (one HIP backend: the same
