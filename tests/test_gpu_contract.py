@@ -273,6 +273,47 @@ def test_graph_replay_matches_eager(ctx, prec):
     plan.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp64_exact", "fp32"])
+def test_concurrent_tile_classes(ctx, prec):
+    """A C-stationary plan with several tile classes runs them on up to four
+    streams (fork and join by events): eager on a created stream, eager on
+    the null stream and graph replays (with eager work between them) all give
+    the oracle's result, FP64 bit for bit."""
+    rng = np.random.default_rng(131)
+    expr = "ik,kj->ij"
+    bs_i = [16, 50, 64, 100, 7, 130]
+    bs_k = [3, 16, 37, 64]
+    bs_j = [33, 48, 64, 80, 130, 20, 9, 16, 5]
+    A = random_bst(rng, [bs_i, bs_k], 0.8)
+    B = random_bst(rng, [bs_k, bs_j], 0.8)
+    Cp = orc.BlockSparse([bs_i, bs_j])
+    _, _, cids = orc.tasks(expr, A, B, Cp)
+    Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+    c0 = rng.uniform(-1, 1, len(Ck.data))
+    _, _, _, R = oracle_result(expr, A, B, Ck, alpha=0.75, beta=1.5, c0=c0, mode=0)
+    Ck.data[:] = c0
+    dA, dB, dC = to_device(ctx, A), to_device(ctx, B), to_device(ctx, Ck)
+    plan = make_plan(ctx, expr, dA, dB, dC, prec)
+    classes = plan.execute_timed(0.0, 1.0)  # (C = 1 * C: values unchanged)
+    assert len(classes) >= 3, classes  # (concurrency needs two or more)
+    check = (lambda x: np.array_equal(x, R.data)) if prec in EXACT else (lambda x: normwise(x, R.data) <= TOL[prec])
+    s = sht.Stream()
+    for run in range(4):
+        dC.upload(c0)
+        if run == 0:
+            plan.execute(0.75, 1.5, s)
+            s.synchronize()
+        elif run == 1:
+            plan.execute(0.75, 1.5)
+            sht.call("sht_device_synchronize")
+        else:
+            plan.graph_replay(0.75, 1.5, 1, s)
+            s.synchronize()
+        assert check(dC.download()), run
+    s.close()
+    plan.close()
+
+
 def test_config0_full_size(ctx):
     """ijk,kl->ijl, every mode 256 in 16 blocks of 16, p=0.3 (seeded)."""
     seed = 1000
