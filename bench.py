@@ -176,7 +176,8 @@ def measure_traffic(args, form, schedule, kclass, c_nelem):
         d = tempfile.mkdtemp(prefix="sht_pmc_", dir="/tmp")
         cmd = [rp, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
                os.path.join(ROOT, "bench.py"), "--config", args.config, "--forms", form, "--steps", "1",
-               "--warmup", "0", "--no-cpu", "--no-traffic", "--calib", "--schedule", str(schedule), "--legs", ""]
+               "--warmup", "0", "--no-cpu", "--no-traffic", "--calib", "--schedule", str(schedule), "--legs", "",
+               "--graph-replays", "0"]
         if args.length:
             cmd += ["--length", str(args.length)]
         log("traffic: %s" % " ".join(cmd))
@@ -563,6 +564,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE runs that measure the dominant kernel's HBM bytes")
     ap.add_argument("--graph", action="store_true", help="time hipGraph replays instead of eager steps")
+    ap.add_argument("--graph-replays", type=int, default=50,
+                    help="graph replays of the first form timed beside its eager steps (0: none)")
     ap.add_argument("--calib", action="store_true",
                     help="also run C = 1*C (alpha=0) through the A-stationary path: a plain read+write of C "
                          "(counter calibration)")
@@ -646,6 +649,19 @@ def main():
                 plan.execute(1.0, 0.0, stream)
             stream.synchronize()
             times.append(time.perf_counter() - ts)
+        graph_rep = None
+        if not args.graph and args.graph_replays > 0 and f == forms[0]:
+            # the headline form also as captured-graph replays (one capture,
+            # then replays timed one by one)
+            plan.graph_replay(1.0, 0.0, 1, stream)
+            stream.synchronize()
+            graph_rep = []
+            for _ in range(args.graph_replays):
+                ts = time.perf_counter()
+                plan.graph_replay(1.0, 0.0, 1, stream)
+                stream.synchronize()
+                graph_rep.append(time.perf_counter() - ts)
+            graph_rep = np.array(graph_rep)
         timed = plan.execute_timed(1.0, 0.0, stream)
         if args.calib:
             sched = plan.get_schedule()
@@ -657,7 +673,7 @@ def main():
             log("calibration: C = 1*C over %d elements (%.3e bytes read, same written)" % (Cf.nelem(),
                                                                                          8.0 * Cf.nelem()))
         times = np.array(times)
-        results[f] = {"stats": st, "times": times, "launches": timed, "plan": plan}
+        results[f] = {"stats": st, "times": times, "launches": timed, "plan": plan, "graph": graph_rep}
         Ctens[f] = Cf
         log("%s: median %.3f ms -> %.1f GFLOP/s; launches %s" % (
             f, 1e3 * np.median(times), st["flops"] / np.median(times) / 1e9, timed))
@@ -781,6 +797,10 @@ def main():
                         "schedule": stf.get("schedule"), "variant": stf.get("variant"),
                         "autotune_ms": stf.get("autotune_ms"),
                         "autotuned_vs_fixed_tile": stf.get("autotuned_vs_fixed_tile"),
+                        "graph_replays": None if results[f]["graph"] is None else {
+                            "n": len(results[f]["graph"]),
+                            "ms_p50": round(1e3 * float(np.percentile(results[f]["graph"], 50)), 4),
+                            "ms_p99": round(1e3 * float(np.percentile(results[f]["graph"], 99)), 4)},
                         "parity": parity.get(f)}
     out = {
         "metric": "contraction GFLOP/s FP64 vs batched/FP32 at 1e-6 rel err; p50/p99 latency",
@@ -805,6 +825,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if forms_out[main_f].get("graph_replays"):
+        out["latency_ms"]["graph_p50"] = forms_out[main_f]["graph_replays"]["ms_p50"]
+        out["latency_ms"]["graph_p99"] = forms_out[main_f]["graph_replays"]["ms_p99"]
     c_nelem = Ctens[main_f].nelem()
     for f in forms:
         results[f]["plan"].close()
