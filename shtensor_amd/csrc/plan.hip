@@ -1447,6 +1447,9 @@ sht_status sht_plan_graph_replay(sht_plan* P, double alpha, double beta, int rep
       if (P->graph_def) SHT_HIP(hipGraphDestroy(P->graph_def));
       P->graph = nullptr;
       P->graph_def = nullptr;
+      // the side streams of concurrent C-stationary classes exist before the
+      // capture starts (no object creation inside a capture)
+      if (P->schedule == 0 && sht::cs_streams(P) > 1) sht::ensure_side_streams(P, sht::cs_streams(P) - 1);
       SHT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
         sht::enqueue(P, alpha, beta, s);
