@@ -963,6 +963,7 @@ void build_as_schedule_impl(sht_plan* P)
   int64_t* a_off = A->d_offsets;
   P->d_as_sweep = static_cast<AsSweep*>(arena.allocate(std::max<int64_t>(nsweep, 1) * sizeof(AsSweep)));
   P->d_as_ccol = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(16 * nslot, 1) * sizeof(int64_t)));
+  P->as_nccol = 16 * nslot;
   if (narrow) P->d_as_sweep4 = static_cast<AsSweep*>(arena.allocate(std::max<int64_t>(nsweep4, 1) * sizeof(AsSweep)));
   if (nseg) {
     hipLaunchKernelGGL(k_as_sweeps, dim3(blocks_for(nseg)), dim3(256), 0, st, nseg, seg_a, seg_panel, seg_nslots,
@@ -1125,6 +1126,9 @@ bool as_supported(const sht_plan* P)
   return vmax < static_cast<double>(1 << 28);
 }
 
+}  // namespace
+
+// (also used by the serving loop's merged A-stationary launches)
 BtFormat bt_format(const sht_plan* P)
 {
   return P->prec == SHT_PREC_FP32 ? BtFormat::F32T : (P->prec == SHT_PREC_BF16X3 ? BtFormat::BF16X3 : BtFormat::F64T);
@@ -1140,6 +1144,8 @@ AsClass as_class(const sht_plan* P)
     default: return wide ? AsClass::Mfma64G : (narrow ? AsClass::Mfma64N : AsClass::Mfma64);
   }
 }
+
+namespace {
 
 // schedule 3 (narrow sweeps) exists for the FP64 and FP32 MFMA forms
 bool narrow_supported(const sht_plan* P) { return P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32; }

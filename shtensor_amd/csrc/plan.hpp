@@ -169,6 +169,9 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s);
 void launch_as_bf3(AsClass k, const AsArgs& a, hipStream_t s);  // the split-BF16 classes
 // waves per workgroup of a class (its units come grouped that way)
 int as_group_waves(AsClass k);
+// The A-stationary kernel class and panel format of a plan's schedule.
+AsClass as_class(const sht_plan* P);
+BtFormat bt_format(const sht_plan* P);
 // C = beta ? beta*C : 0 over nelem doubles (the A-stationary kernel accumulates)
 void launch_scale(double* c, int64_t nelem, double beta, int beta_zero, hipStream_t s);
 
@@ -246,7 +249,7 @@ struct sht_plan {
   sht::BtRec* d_bt_rec = nullptr;
   void* d_bt = nullptr;
   int64_t bt_nrec = 0, bt_total = 0;
-  int64_t as_nunits = 0, as_nunitsw = 0, as_nunits4 = 0, as_nsweep = 0, as_npanel = 0;
+  int64_t as_nunits = 0, as_nunitsw = 0, as_nunits4 = 0, as_nsweep = 0, as_npanel = 0, as_nccol = 0;
   bool as_all_touched = false;  // every C block has a task (first sweeps may initialise C)
   double* d_as_zero = nullptr;  // kAsZeroLen zeros
   // host wall time of the build steps (task list + C pattern, C-stationary

@@ -18,8 +18,10 @@
 //     per (precision, alpha, beta) into one grouped launch per kernel class
 //     (task / C-block records rebased to the arena base, so one launch walks
 //     tiles of many contractions of different orders, shapes and
-//     sparsities); a request whose plan picked the A-stationary schedule runs
-//     its own launch.  Batches go round-robin to kServeStreams streams, so a
+//     sparsities); requests whose plans picked the grouped A-stationary
+//     FP64/FP32 schedule are merged the same way into one launch of that
+//     kernel (sweep, column and unit records rebased); other A-stationary
+//     plans run their own launch.  Batches go round-robin to kServeStreams streams, so a
 //     batch admitted while others run overlaps them on the device.
 //   * Completion: an event per batch; requests complete when it fires.
 //   * Cancellation: a request not yet launched can be cancelled.
@@ -68,6 +70,44 @@ __global__ void k_rebase_tiles(TileRec* dst, const TileRec* src, int64_t n, int3
   if (i >= n) return;
   TileRec r = src[i];
   r.cb += dcb;
+  dst[i] = r;
+}
+
+// A-stationary records of one request moved into a merged batch: element
+// offsets rebased to the arena base (A and C in doubles, the panel in its
+// own element type), sweep column entries to the batch's column array.
+__global__ void k_rebase_sweeps(AsSweep* dst, const AsSweep* src, int64_t n, int64_t da, int64_t db, int64_t dc0)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  AsSweep r = src[i];
+  r.a_off += da;
+  r.b_off += db;
+  r.c0 += dc0;
+  dst[i] = r;
+}
+
+// Column entries: an offset v becomes v + dc, -1 stays; a first-touch
+// marker -(v + 2) keeps its meaning (rebased) when the request's first
+// sweeps start from zeros, and becomes the plain offset otherwise (its C was
+// scaled by beta before the launch and is read like any other column).
+__global__ void k_rebase_ccol(int64_t* dst, const int64_t* src, int64_t n, int64_t dc, int keep_first)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t e = src[i];
+  dst[i] = e >= 0 ? e + dc : (e == -1 ? e : (keep_first ? -((-e - 2) + dc) - 2 : (-e - 2) + dc));
+}
+
+__global__ void k_rebase_units(AsUnit* dst, const AsUnit* src, int64_t n, int64_t npad, int32_t dsw)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  AsUnit r{0, 0, 0, 0};  // (past the request's units: empty filler units)
+  if (i < n) {
+    r = src[i];
+    r.sweep0 += dsw;
+  }
   dst[i] = r;
 }
 
@@ -245,6 +285,97 @@ void launch_group(sht_server* S, const std::vector<Request*>& g, Batch& b, hipSt
   for (Request* r : g) enqueue_stage_out(r->plan, s);
 }
 
+// Requests whose plans run the grouped A-stationary FP64 or FP32 kernel can
+// share one launch.
+bool as_mergeable(const Request* r)
+{
+  const sht_plan* P = r->plan;
+  return P->schedule == 2 && (r->prec == SHT_PREC_FP64 || r->prec == SHT_PREC_FP32);
+}
+
+// Launch one merged batch of grouped A-stationary requests (all share
+// prec/alpha/beta) on stream s: each request's C = beta * C pass (unless its
+// first sweeps start from zeros) and panel conversion, then one launch over
+// the concatenated unit lists (each request's list padded to whole rounds of
+// the eight XCD queues, so its XCD-aware order is kept).
+void launch_group_as(sht_server* S, const std::vector<Request*>& g, Batch& b, hipStream_t s)
+{
+  Arena& arena = S->ctx->arena;
+  const char* base = static_cast<const char*>(S->ctx->d_base);
+  const Request* first = g.front();
+  const bool f32 = first->prec == SHT_PREC_FP32;
+  const int64_t tb = f32 ? 4 : 8;  // panel element bytes
+  auto rel = [base](const void* p, int64_t w) {
+    return static_cast<int64_t>((static_cast<const char*>(p) - base) / w);
+  };
+  const AsClass cls = as_class(first->plan);
+  const int64_t w = as_group_waves(cls);
+  const int64_t round = 8 * w;  // units per round of the eight XCD queues
+  uint64_t ua, ub;
+  std::memcpy(&ua, &first->alpha, 8);
+  std::memcpy(&ub, &first->beta, 8);
+  const bool az = ua == 0, bz = ub == 0;
+  int64_t nsw = 0, ncol = 0, nun = 0;
+  for (Request* r : g) {
+    nsw += r->plan->as_nsweep;
+    ncol += r->plan->as_nccol;
+    nun += (r->plan->as_nunitsw + round - 1) / round * round;
+  }
+  SHT_REQUIRE(nsw < (int64_t(1) << 31) && nun < (int64_t(1) << 31), SHT_ERR_UNSUPPORTED, "batch too large");
+  AsSweep* sweeps = static_cast<AsSweep*>(arena.allocate(std::max<int64_t>(nsw, 1) * sizeof(AsSweep)));
+  b.scratch.push_back(sweeps);
+  int64_t* ccol = static_cast<int64_t*>(arena.allocate(std::max<int64_t>(ncol, 1) * sizeof(int64_t)));
+  b.scratch.push_back(ccol);
+  AsUnit* units = static_cast<AsUnit*>(arena.allocate(std::max<int64_t>(nun, 1) * sizeof(AsUnit)));
+  b.scratch.push_back(units);
+  const double* zero = nullptr;
+  int64_t s0 = 0, c0 = 0, u0 = 0;
+  for (Request* r : g) {
+    sht_plan* P = r->plan;
+    enqueue_stage_in(P, first->alpha, first->beta, s);
+    const void* Ad = P->permA ? P->d_Ap : P->A->d_data;
+    const void* Bd = P->permB ? P->d_Bp : P->B->d_data;
+    void* Cd = P->permC ? P->d_Cp : P->C->d_data;
+    const bool zinit = bz && !az && P->as_all_touched;
+    if (zinit) zero = P->d_as_zero;
+    if (!zinit) launch_scale(static_cast<double*>(Cd), P->C->nelem, first->beta, bz ? 1 : 0, s);
+    if (!az) {
+      launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, P->bt_total, s);
+      if (P->as_nsweep)
+        hipLaunchKernelGGL(k_rebase_sweeps, dim3(nblk(P->as_nsweep)), dim3(256), 0, s, sweeps + s0, P->d_as_sweep,
+                           P->as_nsweep, rel(Ad, 8), rel(P->d_bt, tb), c0);
+      if (P->as_nccol)
+        hipLaunchKernelGGL(k_rebase_ccol, dim3(nblk(P->as_nccol)), dim3(256), 0, s, ccol + c0, P->d_as_ccol,
+                           P->as_nccol, rel(Cd, 8), zinit ? 1 : 0);
+      const int64_t npad = (P->as_nunitsw + round - 1) / round * round;
+      if (npad)
+        hipLaunchKernelGGL(k_rebase_units, dim3(nblk(npad)), dim3(256), 0, s, units + u0, P->d_as_unitw,
+                           P->as_nunitsw, npad, static_cast<int32_t>(s0));
+      SHT_HIP(hipGetLastError());
+      s0 += P->as_nsweep;
+      c0 += P->as_nccol;
+      u0 += npad;
+    }
+  }
+  if (!az && u0 > 0) {
+    AsArgs a{};
+    a.A = base;
+    a.B = base;
+    a.C = const_cast<char*>(base);
+    a.bt_total = 0;
+    a.sweeps = sweeps;
+    a.ccol = ccol;
+    a.units = units;
+    a.nunits = u0;
+    a.alpha = first->alpha;
+    a.alpha_one = first->alpha == 1.0 ? 1 : 0;
+    a.zero = zero;
+    launch_as(cls, a, s);
+    S->nlaunches++;
+  }
+  for (Request* r : g) enqueue_stage_out(r->plan, s);
+}
+
 // Complete the requests of b (status already set on failures): release
 // plans, merged records and footprints, wake waiters.
 void finish(sht_server* S, Batch& b)
@@ -406,20 +537,22 @@ void dispatcher_loop(sht_server* S)
       }
     }
     if (!done.reqs.empty()) finish(S, done);
-    // C-stationary plans merge per (precision, alpha, beta); A-stationary
-    // plans run alone
+    // C-stationary plans merge per (precision, alpha, beta), and so do
+    // plans that run the grouped A-stationary FP64/FP32 kernel; other
+    // A-stationary plans run alone
     std::vector<std::vector<Request*>> groups;
     {
-      std::map<std::tuple<int, uint64_t, uint64_t>, std::vector<Request*>> merged;
+      std::map<std::tuple<int, int, uint64_t, uint64_t>, std::vector<Request*>> merged;
       for (Request* r : runnable) {
-        if (r->plan->schedule >= 1) {
+        const bool as = r->plan->schedule >= 1;
+        if (as && !as_mergeable(r)) {
           groups.push_back({r});
           continue;
         }
         uint64_t ua, ub;
         std::memcpy(&ua, &r->alpha, 8);
         std::memcpy(&ub, &r->beta, 8);
-        merged[std::make_tuple(static_cast<int>(r->prec), ua, ub)].push_back(r);
+        merged[std::make_tuple(as ? 1 : 0, static_cast<int>(r->prec), ua, ub)].push_back(r);
       }
       for (auto& kv : merged) groups.push_back(std::move(kv.second));
     }
@@ -432,6 +565,8 @@ void dispatcher_loop(sht_server* S)
         if (g.size() == 1 && g[0]->plan->schedule >= 1) {
           enqueue(g[0]->plan, g[0]->alpha, g[0]->beta, s);
           S->nlaunches++;
+        } else if (g[0]->plan->schedule >= 1) {
+          launch_group_as(S, g, b, s);
         } else {
           launch_group(S, g, b, s);
         }

@@ -315,3 +315,51 @@ def test_plan_out_of_arena_is_requeued():
         assert requeued >= 1
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("prec", [sht.PREC_FP64, sht.PREC_FP32])
+@pytest.mark.parametrize("beta", [0.0, 0.5])
+def test_a_stationary_requests_merged(ctx, prec, beta):
+    """Requests whose plans pick the grouped A-stationary schedule (A much
+    larger than B, long contracted blocks) share one launch of that kernel
+    (sweep, column and unit records rebased to the arena base); every result
+    equals the same contraction run alone, bit for bit, with beta = 0 (first
+    sweeps start from zeros) and beta != 0 (C scaled first)."""
+    rng = np.random.default_rng(41 + int(beta * 10) + int(prec))
+    expr = "ijkl,klm->ijm"
+    reqs = []
+    for r in range(4):
+        bs = {"i": rng.integers(10, 40, 3).tolist(), "j": rng.integers(2, 9, 3).tolist(),
+              "k": rng.integers(8, 13, 3).tolist(), "l": rng.integers(8, 13, 2).tolist(),
+              "m": rng.integers(5, 20, 2).tolist()}
+        A = random_bst(rng, [bs[x] for x in "ijkl"], 0.7)
+        B = random_bst(rng, [bs[x] for x in "klm"], 0.5)
+        Cp = orc.BlockSparse([bs[x] for x in "ijm"])
+        _, _, cids = orc.tasks(expr, A, B, Cp)
+        Ck = orc.BlockSparse(Cp.block_sizes).reserve_ids(cids)
+        Ck.data[:] = rng.uniform(-1, 1, len(Ck.data))
+        reqs.append((A, B, Ck))
+    alone, dev = [], []
+    for A, B, Ck in reqs:
+        dA, dB, dC = to_device(ctx, A), to_device(ctx, B), to_device(ctx, Ck)
+        plan = sht.Plan(ctx, expr, dA, dB, dC, prec)
+        assert plan.get_schedule() == 2  # the default picks the grouped A-stationary form
+        plan.execute(0.75, beta)
+        sht.call("sht_device_synchronize")
+        plan.close()
+        alone.append(dC.download())
+        dev.append((dA, dB, Ck))
+    srv = sht.Server(ctx, 2 << 30, max_batch=16)
+    outs, ids = [], []
+    for dA, dB, Ck in dev:
+        dC = to_device(ctx, Ck)
+        outs.append(dC)
+        ids.append(srv.submit(expr, 0.75, dA, dB, beta, dC, prec))
+    for i in ids:
+        srv.wait(i)
+    st = srv.stats()
+    srv.close()
+    assert st["done"] == len(reqs)
+    assert st["max_merged"] >= 2, st
+    for ref, dC in zip(alone, outs):
+        assert np.array_equal(dC.download(), ref)
