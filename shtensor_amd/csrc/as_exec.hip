@@ -35,6 +35,8 @@
 //    product, FP64 across blocks.
 //  * k_as_exact: the fma chain with one row per lane (bitwise as well).
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "plan.hpp"
@@ -551,25 +553,13 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   }
 }
 
-// Workgroups of W waves over W row chunks of one block row (the LDS-shared
-// panel path; F: 0 = FP64 MFMA, 1 = FP32 MFMA).  Every wave of a workgroup
-// runs the same sweep list, ghosts included, so the workgroup barriers match.
+// One workgroup task of the grouped kernel: the W units (row chunks) of one
+// block row walk the row's sweep list.
 template <int F, int W, bool A1, int AD>
-__global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
+__device__ __attribute__((always_inline)) inline void asl_task(const AsArgs& p, int wave, int lane, int li, int lk,
+                                                               int nsw, int s0, int r0, bool ghost, int64_t* cbs,
+                                                               typename BElem<F>::type (*stage)[kAsSweepSlots][64])
 {
-  static_assert(AD == 4 || AD == 8, "A ring of 4 or 8 steps");
-  typedef typename BElem<F>::type TB;
-  const int wave = uni(static_cast<int>(threadIdx.x >> 6));
-  const int64_t ui = static_cast<int64_t>(blockIdx.x) * W;
-  if (ui >= p.nunits) return;  // (the unit list is padded to whole workgroups)
-  const AsUnit u = p.units[ui + wave];
-  const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
-  const bool ghost = uni(u.pad) != 0;
-  if (nsw == 0) return;  // a whole-workgroup filler (every wave of it)
-  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  __shared__ int64_t cbs_all[W][128];
-  __shared__ TB stage[kAslStages][kAsSweepSlots][64];
-  int64_t* cbs = cbs_all[wave];
   SweepU cur = to_uni(p.sweeps[s0]);
   SweepU nxt = to_uni(p.sweeps[s0 + min(1, nsw - 1)]);
   AsSweep raw2 = p.sweeps[s0 + min(2, nsw - 1)];
@@ -604,6 +594,44 @@ __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
     cur = nxt;
     nxt = to_uni(raw2);
     raw2 = p.sweeps[s0 + min(j + 3, nsw - 1)];
+  }
+}
+
+// Workgroups of W waves over W row chunks of one block row (the LDS-shared
+// panel path; F: 0 = FP64 MFMA, 1 = FP32 MFMA).  Every wave of a workgroup
+// runs the same sweep list, ghosts included, so the workgroup barriers match.
+// Persistent: the grid holds as many workgroups as the device keeps resident
+// (a multiple of eight); workgroup b serves XCD queue b % 8 of the unit list
+// (workgroup task t of queue x is the list's workgroup 8t + x) and takes its
+// next task from that queue's counter until the queue is empty.  The task
+// index is read by every wave after a barrier, so every exit and skip is
+// taken by the whole workgroup.
+template <int F, int W, bool A1, int AD>
+__global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
+{
+  static_assert(AD == 4 || AD == 8, "A ring of 4 or 8 steps");
+  typedef typename BElem<F>::type TB;
+  const int wave = uni(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  __shared__ int64_t cbs_all[W][128];
+  __shared__ TB stage[kAslStages][kAsSweepSlots][64];
+  __shared__ unsigned long long next_task;
+  int64_t* cbs = cbs_all[wave];
+  const unsigned x = blockIdx.x & 7u;
+  const int64_t ntask = (p.nunits / W + 7) / 8;  // workgroup tasks per queue
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous task's LDS
+    if (threadIdx.x == 0) next_task = atomicAdd(p.qctr + x, 1ull);
+    __syncthreads();
+    const int64_t t = uni(static_cast<int64_t>(next_task));
+    if (t >= ntask) break;
+    const int64_t ui = (8 * t + x) * W;
+    if (ui >= p.nunits) continue;  // (the last round of the queues is short)
+    const AsUnit u = p.units[ui + wave];
+    const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
+    const bool ghost = uni(u.pad) != 0;
+    if (nsw == 0) continue;  // a whole-workgroup filler (every wave of it)
+    asl_task<F, W, A1, AD>(p, wave, lane, li, lk, nsw, s0, r0, ghost, cbs, stage);
   }
 }
 
@@ -1121,12 +1149,54 @@ int as_group_waves(AsClass k)
   }
 }
 
+namespace {
+
+__global__ void k_clear_qctr(unsigned long long* c)
+{
+  if (threadIdx.x < 8) c[threadIdx.x] = 0;
+}
+
+// Grid of the persistent grouped kernel f: the workgroups the device keeps
+// resident (a multiple of eight, one set per XCD queue), at most one per
+// workgroup task.  The occupancy is queried once per device and kernel.
+template <class Kern>
+unsigned persistent_grid(Kern f, int threads, int64_t ntasks)
+{
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> cache;
+  int dev = 0;
+  SHT_HIP(hipGetDevice(&dev));
+  int resident = 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(dev, reinterpret_cast<const void*>(f));
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+      int per_cu = 0, cus = 0;
+      SHT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, 0));
+      SHT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      it = cache.emplace(key, std::max(8, per_cu * cus / 8 * 8)).first;
+    }
+    resident = it->second;
+  }
+  const int64_t want = (ntasks + 7) / 8 * 8;
+  return static_cast<unsigned>(std::min<int64_t>(want, resident));
+}
+
+}  // namespace
+
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
 {
   if (a.nunits <= 0) return;
   constexpr int G = kAsGroupWaves;
   const dim3 g1(static_cast<unsigned>(a.nunits)), b1(64);
-  const dim3 gg(static_cast<unsigned>((a.nunits + G - 1) / G)), bg(64 * G);
+  const dim3 bg(64 * G);
+  const int64_t nwg = (a.nunits + G - 1) / G;  // grouped workgroup tasks
+  auto persistent = [&](auto kern) {
+    SHT_REQUIRE(a.qctr, SHT_ERR_STATE, "grouped kernel without work counters");
+    hipLaunchKernelGGL(k_clear_qctr, dim3(1), dim3(64), 0, s, a.qctr);
+    hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, 64 * G, nwg)), bg, 0, s, a);
+  };
   static_assert(kAsUnitRows == 32 && kAsUnitRowsW == 64, "unit rows: two MFMA tiles; 64 lanes");
   static_assert(kAsSweepSlots == 8, "sweep slots");
   // the grouped classes read W unit records per workgroup
@@ -1137,11 +1207,11 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
       else hipLaunchKernelGGL((k_asp<2, 8, 0, 1, false>), g1, b1, 0, s, a);
       break;
     case AsClass::Mfma64G:
-      if (a.alpha_one) hipLaunchKernelGGL((k_asl<0, G, true, 4>), gg, bg, 0, s, a);
-      else hipLaunchKernelGGL((k_asl<0, G, false, 4>), gg, bg, 0, s, a);
+      if (a.alpha_one) persistent(k_asl<0, G, true, 4>);
+      else persistent(k_asl<0, G, false, 4>);
       break;
     case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Mfma32G: hipLaunchKernelGGL((k_asl<1, G, false, 4>), gg, bg, 0, s, a); break;
+    case AsClass::Mfma32G: persistent(k_asl<1, G, false, 4>); break;
     case AsClass::Bf16x3:
     case AsClass::Bf16x3G: launch_as_bf3(k, a, s); break;
     case AsClass::Mfma64N:

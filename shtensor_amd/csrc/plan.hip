@@ -1092,6 +1092,7 @@ void build_as_schedule_impl(sht_plan* P)
     P->as_all_touched = std::all_of(h.begin(), h.end(), [](char c) { return c != 0; });
     P->d_as_zero = static_cast<double*>(arena.allocate(kAsZeroLen * sizeof(double)));
     SHT_HIP(hipMemsetAsync(P->d_as_zero, 0, kAsZeroLen * sizeof(double), st));
+    if (!P->d_as_qctr) P->d_as_qctr = static_cast<unsigned long long*>(arena.allocate(8 * sizeof(unsigned long long)));
   }
   SHT_HIP(hipStreamSynchronize(st));
   P->bt_total = bt_total;
@@ -1274,6 +1275,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.alpha = alpha;
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       g.zero = zinit ? P->d_as_zero : nullptr;
+      g.qctr = P->d_as_qctr;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
       launch_as(cls, g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));

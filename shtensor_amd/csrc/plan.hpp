@@ -149,6 +149,10 @@ struct AsArgs {
   // column starts from zeros read here (kAsZeroLen doubles) instead of C,
   // and no C = beta * C pass runs before the sweeps
   const double* zero;
+  // the grouped FP64/FP32 kernel is persistent: its workgroups take
+  // workgroup tasks from the eight XCD queues of the unit list through these
+  // eight counters (cleared by launch_as before every launch)
+  unsigned long long* qctr;
 };
 constexpr int kAsZeroLen = 256;
 // Kernel classes.  The MFMA classes run 32-row units (two 16-row MFMA tiles
@@ -252,6 +256,7 @@ struct sht_plan {
   int64_t as_nunits = 0, as_nunitsw = 0, as_nunits4 = 0, as_nsweep = 0, as_npanel = 0, as_nccol = 0;
   bool as_all_touched = false;  // every C block has a task (first sweeps may initialise C)
   double* d_as_zero = nullptr;  // kAsZeroLen zeros
+  unsigned long long* d_as_qctr = nullptr;  // work counters of the persistent grouped kernel
   // host wall time of the build steps (task list + C pattern, C-stationary
   // tiles, A-stationary schedule), ms
   double ms_tasks = 0.0, ms_cstat = 0.0, ms_astat = 0.0;
@@ -268,7 +273,7 @@ struct sht_plan {
                     static_cast<void*>(d_as_ccol), static_cast<void*>(d_as_unit),
                     static_cast<void*>(d_as_unitw), static_cast<void*>(d_as_sweep4),
                     static_cast<void*>(d_as_unit4), static_cast<void*>(d_bt_rec), d_bt,
-                    static_cast<void*>(d_as_zero)})
+                    static_cast<void*>(d_as_zero), static_cast<void*>(d_as_qctr)})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);

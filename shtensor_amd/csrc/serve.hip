@@ -328,6 +328,8 @@ void launch_group_as(sht_server* S, const std::vector<Request*>& g, Batch& b, hi
   b.scratch.push_back(ccol);
   AsUnit* units = static_cast<AsUnit*>(arena.allocate(std::max<int64_t>(nun, 1) * sizeof(AsUnit)));
   b.scratch.push_back(units);
+  auto* qctr = static_cast<unsigned long long*>(arena.allocate(8 * sizeof(unsigned long long)));
+  b.scratch.push_back(qctr);
   const double* zero = nullptr;
   int64_t s0 = 0, c0 = 0, u0 = 0;
   for (Request* r : g) {
@@ -370,6 +372,7 @@ void launch_group_as(sht_server* S, const std::vector<Request*>& g, Batch& b, hi
     a.alpha = first->alpha;
     a.alpha_one = first->alpha == 1.0 ? 1 : 0;
     a.zero = zero;
+    a.qctr = qctr;
     launch_as(cls, a, s);
     S->nlaunches++;
   }
