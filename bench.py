@@ -534,6 +534,36 @@ def leg_kernel_call(sht, orc, nops=5898):
     k.close()
     for x in (da, db, dc):
         x.free()
+    # larger square blocks through the same entry point (bitwise on every op)
+    out["shapes"] = []
+    for sz, n in ((64, 4096), (128, 512)):
+        sq = [sz, sz]
+        rng = np.random.default_rng(sz)
+        a = rng.uniform(-1, 1, n * sz * sz)
+        b = rng.uniform(-1, 1, n * sz * sz)
+        c = np.zeros(n * sz * sz)
+        k = sht.Kernel("ik,kj->ij", sq, sq, sq, 1.0, 0.0)
+        da, db, dc = sht.DeviceArray(a), sht.DeviceArray(b), sht.DeviceArray(c)
+        for _ in range(3):
+            k.call(da, db, dc, n, stream)
+        stream.synchronize()
+        ts = []
+        for _ in range(10):
+            t = time.perf_counter()
+            k.call(da, db, dc, n, stream)
+            stream.synchronize()
+            ts.append(time.perf_counter() - t)
+        m = float(np.median(ts))
+        got = dc.numpy()
+        want = np.zeros_like(c)
+        orc.kernel_xmm("ik,kj->ij", sq, sq, sq, 1.0, 0.0, a, b, want, n)
+        fl = 2.0 * sz ** 3 * n
+        out["shapes"].append({"shape": "ik,kj->ij %dx%dx%d" % (sz, sz, sz), "nb_ops": n, "gpu_ms": round(1e3 * m, 4),
+                              "gflops": round(fl / m / 1e9, 2), "gbs": round(24.0 * sz * sz * n / m / 1e9, 1),
+                              "bitwise": bool(np.array_equal(got, want))})
+        k.close()
+        for x in (da, db, dc):
+            x.free()
     stream.close()
     log("kernel_call leg: %s" % out)
     return out

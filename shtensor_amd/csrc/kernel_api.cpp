@@ -37,6 +37,7 @@ struct sht_kernel {
   hipFunction_t jit_fn = nullptr;  // owned by the process-wide cache
   bool jit_mfma = false;           // the specialised kernel is the MFMA form
   int64_t jit_wpo = 1;             // its workgroups per op
+  int jit_nw = 1;                  // and waves per workgroup
   // device staging for host-memory operands (grown on demand, freed with the kernel)
   void* d_stage = nullptr;
   size_t stage_bytes = 0;
@@ -91,18 +92,22 @@ std::unordered_map<std::string, JitEntry> g_jit;
 // Launch shape of a specialised kernel.  FMA form: one lane per C element
 // of one op (grid-stride).  MFMA form (FP64, at least 16 rows and 4
 // contracted indices): one wave per group of R 16-row tiles x one 16-column
-// tile of one op, four waves (four row groups of the same op and column tile)
-// per workgroup.
+// tile of one op; the waves of a workgroup (at most four) cover WR row
+// groups x WC column tiles of the same op and share the WC column tiles' B
+// panel through LDS.
 struct JitShape {
   bool mfma = false;
   int R = 1;      // 16-row tiles per wave
   int NG = 1;     // row groups per op
   int NU = 1;     // 16-column tiles per op
+  int WR = 1;     // waves of a workgroup along the row groups
+  int WC = 1;     // waves of a workgroup along the column tiles
+  int NW = 1;     // waves per workgroup (WR x WC)
   int WPO = 1;    // workgroups per op
-  bool bl = false;  // the column tile's B panel staged in LDS
+  bool bl = false;  // the workgroup's B panel (16 WC columns) staged in LDS
 };
 
-constexpr int kKcWaves = 4;  // waves per workgroup of the MFMA form
+constexpr int kKcWaves = 4;  // most waves per workgroup of the MFMA form
 
 // Device staging of host operands: A, B, C of nb_ops ops, 256-byte aligned.
 struct StageLayout {
@@ -120,8 +125,17 @@ JitShape jit_shape(const sht_kernel& k)
   js.R = std::min(4, (k.M + 15) / 16);
   js.NG = (k.M + 16 * js.R - 1) / (16 * js.R);
   js.NU = (k.N + 15) / 16;
-  js.WPO = (js.NG + kKcWaves - 1) / kKcWaves * js.NU;
-  js.bl = static_cast<int64_t>(k.K) * 16 * 8 <= 32768;
+  // Workgroups hold only waves with work: WR row groups x WC column tiles.
+  // Blocks of one row group take two column tiles per workgroup (four made
+  // the staged B panel too large for residency: 64 x 64 x 64 ops 17.7 TF
+  // with two, 13.1 with four, 16.5 with one); taller blocks fill four waves
+  // (128 x 128 x 128: 17.8 -> 27.4 TF against four row-group waves per
+  // column tile).
+  js.WR = std::min(kKcWaves, js.NG);
+  js.WC = std::min(js.NG == 1 ? 2 : kKcWaves / js.WR, js.NU);
+  js.NW = js.WR * js.WC;
+  js.WPO = (js.NG + js.WR - 1) / js.WR * ((js.NU + js.WC - 1) / js.WC);
+  js.bl = static_cast<int64_t>(k.K) * 16 * js.WC * 8 <= 32768;
   return js;
 }
 
@@ -180,7 +194,8 @@ std::string jit_source(const sht_kernel& k, bool ca, bool cb, bool cc)
   // past M and columns past N read clamped (valid) addresses and are not
   // stored.
   s << "#define R " << js.R << "\n#define NG " << js.NG << "\n#define NU " << js.NU << "\n#define WPO " << js.WPO
-    << "\n#define BL " << (js.bl ? 1 : 0) << "\n#define NW " << kKcWaves << "\n"
+    << "\n#define WR " << js.WR << "\n#define WC " << js.WC << "\n#define BL " << (js.bl ? 1 : 0)
+    << "\n#define NW " << js.NW << "\n"
     << R"SRC(typedef double v4d __attribute__((ext_vector_type(4)));
 #define K4 (K / 4 * 4)
 __device__ __forceinline__ int aoff(const int* ar, const int* ac, int i, int k) { return CA ? i + k * M : ar[i] + ac[k]; }
@@ -191,29 +206,33 @@ extern "C" __global__ __launch_bounds__(64 * NW) void sht_blockprod(const double
 {
   const int* ar = offs; const int* ac = ar + M; const int* br = ac + K;
   const int* bc = br + K; const int* cr = bc + N; const int* cc = cr + M;
+#define NUB ((NU + WC - 1) / WC)
+#define BW (16 * WC)
 #if BL
-  __shared__ double bs[K * 16];
+  __shared__ double bs[K * BW];
 #endif
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int wr = wave % WR, wc = wave / WR;  // (waves past WR x WC only stage B)
   const long long nwg = nb_ops * WPO;
   for (long long wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
     const long long op = wg / WPO;
     const int rem = (int)(wg - op * WPO);
-    const int u = rem % NU, g = (rem / NU) * NW + wave;
+    const int ub = rem % NUB, g = (rem / NUB) * WR + wr;
+    const int u = ub * WC + wc;
     const double* a = A + op * (long long)(M * K);
     const double* b = B + op * (long long)(K * N);
     double* c = C + op * (long long)(M * N);
     const int j0 = 16 * u;
 #if BL
     __syncthreads();  // the previous item's reads of bs are done
-    for (int x = threadIdx.x; x < K * 16; x += 64 * NW) {
+    for (int x = threadIdx.x; x < K * BW; x += 64 * NW) {
       const int kk = x % K, jj = x / K;
-      bs[kk * 16 + jj] = b[boff(br, bc, kk, min(j0 + jj, N - 1))];
+      bs[kk * BW + jj] = b[boff(br, bc, kk, min(BW * ub + jj, N - 1))];
     }
     __syncthreads();
 #endif
-    if (g >= NG) continue;
+    if (wave >= WR * WC || g >= NG || u >= NU) continue;
     const int i0 = 16 * R * g;
     int ri[R];
 #pragma unroll
@@ -231,7 +250,7 @@ extern "C" __global__ __launch_bounds__(64 * NW) void sht_blockprod(const double
 #pragma unroll 4
       for (int k0 = 0; k0 < K4; k0 += 4) {
 #if BL
-        const double bf = bs[(k0 + lk) * 16 + li];
+        const double bf = bs[(k0 + lk) * BW + 16 * wc + li];
 #else
         const double bf = b[boff(br, bc, k0 + lk, jb)];
 #endif
@@ -370,7 +389,7 @@ void kernel_launch(const sht_kernel& k, const void* a, const void* b, void* c, i
     if (k.jit_mfma) {
       const int64_t nwg = nb_ops * k.jit_wpo;
       g = static_cast<unsigned>(nwg < 1048576 ? nwg : 1048576);
-      bs = 64 * kKcWaves;
+      bs = 64 * k.jit_nw;
     } else {
       const int64_t total = nb_ops * static_cast<int64_t>(k.M) * k.N;
       const int64_t blocks = (total + 255) / 256;
@@ -543,6 +562,7 @@ sht_status sht_kernel_create(const char* expr, const int* sizes_a, int na, const
       const sht::JitShape js = sht::jit_shape(*k);
       k->jit_mfma = js.mfma;
       k->jit_wpo = js.WPO;
+      k->jit_nw = js.NW;
     }
     *out = k.release();
   });

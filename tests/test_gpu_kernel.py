@@ -69,6 +69,10 @@ MFMA_CASES = [
     ("kil,ljk->ij", [6, 20, 5], [5, 9, 6], [20, 9]),
     ("ik,kjl->ilj", [40, 6], [6, 5, 3], [40, 3, 5]),
     ("jik,kmj->im", [6, 24, 8], [8, 19, 6], [24, 19]),
+    # two row groups x two column tiles per workgroup (contiguous and
+    # permuted maps, partial tiles)
+    ("ik,kj->ij", [96, 24], [24, 40], [96, 40]),
+    ("kil,ljk->ij", [6, 80, 5], [5, 35, 6], [80, 35]),
 ]
 
 
