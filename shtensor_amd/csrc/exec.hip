@@ -54,7 +54,7 @@ __global__ __launch_bounds__(64) void k_exact(ExecArgs p)
   __shared__ double Bs[KC * NC];
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // a padding tile of the XCD-interleaved order
-  const CBlkRec cb = p.cblk[tile.cb];
+  const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n, n0 = tile.n0;
   const int nc = min(NC, N - n0);
   const int lane = threadIdx.x;
@@ -75,8 +75,12 @@ __global__ __launch_bounds__(64) void k_exact(ExecArgs p)
     for (int j = 0; j < NC; ++j)
       acc[r][j] = (!p.beta_zero && ok[r] && j < nc) ? Cb[row[r] + static_cast<int64_t>(n0 + j) * M] * p.beta : 0.0;
   if (!p.alpha_zero) {
+    // the first task's record comes with the tile; each next one is loaded
+    // while the current task runs
+    TaskRec trn{tile.a0, tile.b0, tile.k0, 0};
     for (int t = cb.t0; t < cb.t1; ++t) {
-      const TaskRec tr = p.tasks[t];
+      const TaskRec tr = trn;
+      trn = p.tasks[min(t + 1, cb.t1 - 1)];
       const double* Ab = A + tr.a_off;
       const double* Bb = B + tr.b_off;
       const int K = tr.k;
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 {
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // a padding tile of the XCD-interleaved order
-  const CBlkRec cb = p.cblk[tile.cb];
+  const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n;
   const int lane = threadIdx.x;
   const int li = lane & 15, lk = lane >> 4;
@@ -156,8 +160,12 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
       }
   const double apad = pad_a(p.alpha, p.alpha_one != 0);
   if (!p.alpha_zero) {
+    // the first task's record comes with the tile; each next one is loaded
+    // while the current task runs
+    TaskRec trn{tile.a0, tile.b0, tile.k0, 0};
     for (int t = cb.t0; t < cb.t1; ++t) {
-      const TaskRec tr = p.tasks[t];
+      const TaskRec tr = trn;
+      trn = p.tasks[min(t + 1, cb.t1 - 1)];
       const double* Ab = A + tr.a_off;
       const double* Bb = B + tr.b_off;
       const int K = tr.k;
@@ -242,7 +250,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   typedef typename std::conditional<F == 0, double, float>::type TL;
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
-  const CBlkRec cb = p.cblk[tile.cb];
+  const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
     const int ia = min(tile.m0 + rA, M - 1);
     const int jb = min(tile.n0 + jB, N - 1);
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
-    TaskRec tr = p.tasks[tl];
+    TaskRec tr{tile.a0, tile.b0, tile.k0, 0};  // (task tl = t0, carried by the tile record)
     double ra[EA], rb[EB];
     bool ka;       // this lane's A column is inside the task
     int kbb, Kb;   // contracted index of rb[0] and the task's K
@@ -406,7 +414,7 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
 {
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // a padding tile of the XCD-interleaved order
-  const CBlkRec cb = p.cblk[tile.cb];
+  const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n;
   const int lane = threadIdx.x;
   const int li = lane & 15, lk = lane >> 4;
@@ -425,8 +433,12 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
         acc[rt][ct][v] = (!p.beta_zero && i < M && j < N) ? Cb[i + static_cast<int64_t>(j) * M] * p.beta : 0.0;
       }
   if (!p.alpha_zero) {
+    // the first task's record comes with the tile; each next one is loaded
+    // while the current task runs
+    TaskRec trn{tile.a0, tile.b0, tile.k0, 0};
     for (int t = cb.t0; t < cb.t1; ++t) {
-      const TaskRec tr = p.tasks[t];
+      const TaskRec tr = trn;
+      trn = p.tasks[min(t + 1, cb.t1 - 1)];
       const double* Ab = A + tr.a_off;
       const double* Bb = B + tr.b_off;
       const int K = tr.k;
@@ -538,7 +550,7 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
   static_assert(TRt >= 1 && TCt >= 1 && KB >= 2, "tile shape");
   const TileRec tile = p.tiles[blockIdx.x];
   if (tile.pad) return;  // (uniform over the workgroup)
-  const CBlkRec cb = p.cblk[tile.cb];
+  const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -567,7 +579,7 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
     const int ia = min(tile.m0 + ra_, M - 1);
     const int jb = min(tile.n0 + cb_, N - 1);
     int tl = cb.t0, kl = 0;  // load cursor (uniform): task, first contracted index
-    TaskRec tr = p.tasks[tl];
+    TaskRec tr{tile.a0, tile.b0, tile.k0, 0};  // (task tl = t0, carried by the tile record)
     double ra[KA], rb[KB];
     int k0a, k0b, Kl;  // contracted index of ra[0] and of rb[0], the task's K
     bool lastl;   // the loaded chunk is the last of its task (uniform)

@@ -467,6 +467,29 @@ KClass quad_rows(KClass k)
 
 void build_tiles(sht_plan* P);
 
+// Each tile record's copy of its C block record and of its first task's
+// operand offsets (padding tiles stay empty).
+__global__ void k_tile_fill(TileRec* t, int64_t n, const CBlkRec* cblk, const TaskRec* tasks)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TileRec r = t[i];
+  if (r.pad) return;
+  const CBlkRec c = cblk[r.cb];
+  r.c_off = c.c_off;
+  r.m = c.m;
+  r.n = c.n;
+  r.t0 = c.t0;
+  r.t1 = c.t1;
+  if (c.t1 > c.t0) {
+    const TaskRec k = tasks[c.t0];
+    r.a0 = k.a_off;
+    r.b0 = k.b_off;
+    r.k0 = k.k;
+  }
+  t[i] = r;
+}
+
 void build_schedule(sht_plan* P)
 {
   Arena& arena = P->ctx->arena;
@@ -597,7 +620,11 @@ void build_tiles(sht_plan* P)
       for (int n0 = 0; n0 < N; n0 += cols) {
         const double cost =
             static_cast<double>(std::min(kclass_rows(kt), M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
-        per[static_cast<int>(kt)].push_back({TileRec{static_cast<int32_t>(c), m0, n0, 0}, cost, rk});
+        TileRec tr{};
+        tr.cb = static_cast<int32_t>(c);
+        tr.m0 = m0;
+        tr.n0 = n0;
+        per[static_cast<int>(kt)].push_back({tr, cost, rk});
       }
       m0 += kclass_rows(kt);
     }
@@ -644,7 +671,11 @@ void build_tiles(sht_plan* P)
     for (const auto& qq : q) longest = std::max(longest, qq.size());
     const int64_t first = static_cast<int64_t>(all.size());
     for (size_t slot = 0; slot < longest; ++slot)
-      for (int x = 0; x < kXcd; ++x) all.push_back(slot < q[x].size() ? q[x][slot] : TileRec{0, 0, 0, 1});
+      for (int x = 0; x < kXcd; ++x) {
+        TileRec padt{};
+        padt.pad = 1;
+        all.push_back(slot < q[x].size() ? q[x][slot] : padt);
+      }
     while (!all.empty() && static_cast<int64_t>(all.size()) > first && all.back().pad) all.pop_back();
     double ck = 0.0;
     for (const RowSpan& r : rows) ck += r.cost;
@@ -653,6 +684,10 @@ void build_tiles(sht_plan* P)
   if (!all.empty()) {
     P->d_tiles = static_cast<TileRec*>(arena.allocate(all.size() * sizeof(TileRec)));
     SHT_HIP(hipMemcpy(P->d_tiles, all.data(), all.size() * sizeof(TileRec), hipMemcpyHostToDevice));
+    const int64_t n = static_cast<int64_t>(all.size());
+    hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n)), dim3(256), 0, nullptr, P->d_tiles, n, P->d_cblk, P->d_tasks);
+    SHT_HIP(hipGetLastError());
+    SHT_HIP(hipStreamSynchronize(nullptr));
   }
 }
 

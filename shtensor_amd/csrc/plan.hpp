@@ -24,11 +24,20 @@ __device__ inline double pad_a(double alpha, bool alpha_one)
 // One C-stationary tile: rows [m0, m0+rows) and columns [n0, n0+cols) of
 // C block `cb`; the kernel class fixes rows/cols.  pad = 1: a padding tile
 // (no work) that keeps the XCD-interleaved tile order aligned.
+// The tile record carries its C block's record and the first task's
+// operand offsets (filled on the device once the tile list is built), so a
+// tile's first loads wait for one record, not a chain of three.
 struct TileRec {
   int32_t cb;
   int32_t m0;
   int32_t n0;
   int32_t pad;
+  int64_t c_off;   // C block (as CBlkRec)
+  int32_t m, n;
+  int32_t t0, t1;
+  int64_t a0, b0;  // first task (t0): A and B block offsets and its K
+  int32_t k0;
+  int32_t pad2;
 };
 
 // Per C block: element offset, shape and task range (tasks sorted by K key).

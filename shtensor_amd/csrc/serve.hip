@@ -64,12 +64,22 @@ __global__ void k_rebase_cblk(CBlkRec* dst, const CBlkRec* src, int64_t n, int64
   dst[i] = r;
 }
 
-__global__ void k_rebase_tiles(TileRec* dst, const TileRec* src, int64_t n, int32_t dcb)
+// (a tile record carries its C block's record and its first task's operand
+// offsets: rebased like those records)
+__global__ void k_rebase_tiles(TileRec* dst, const TileRec* src, int64_t n, int32_t dcb, int64_t da, int64_t db,
+                               int64_t dc, int32_t dt)
 {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   TileRec r = src[i];
-  r.cb += dcb;
+  if (!r.pad) {
+    r.cb += dcb;
+    r.c_off += dc;
+    r.t0 += dt;
+    r.t1 += dt;
+    r.a0 += da;
+    r.b0 += db;
+  }
   dst[i] = r;
 }
 
@@ -254,7 +264,8 @@ void launch_group(sht_server* S, const std::vector<Request*>& g, Batch& b, hipSt
     for (const auto& L : P->launches) {
       const int k = static_cast<int>(L.k);
       hipLaunchKernelGGL(k_rebase_tiles, dim3(nblk(L.count)), dim3(256), 0, s, tiles[k] + tfill[k],
-                         P->d_tiles + L.first, L.count, static_cast<int32_t>(c0));
+                         P->d_tiles + L.first, L.count, static_cast<int32_t>(c0), rel(Ad), rel(Bd), rel(Cd),
+                         static_cast<int32_t>(t0));
       tfill[k] += L.count;
     }
     SHT_HIP(hipGetLastError());
