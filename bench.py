@@ -64,6 +64,12 @@ CONFIGS = {
 }
 
 
+def f32_operands(form):
+    """The FP32 and split-BF16 forms read FP32 operand tensors unless the
+    form name asks for the FP64 ones (_f64in)."""
+    return form.split("_")[0] in ("fp32", "bf16x3") and "_f64in" not in form
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -446,9 +452,13 @@ def leg_c3(sht, orc, stream):
             expr, sizes, eabc, A, B, Cc = build_tensors(sht, ctx, cfg)
             R = oracle_full(orc, cfg, sizes, eabc)
             rec = {}
-            for name, pr in (("fp64", sht.PREC_FP64), ("fp32", sht.PREC_FP32), ("bf16x3", sht.PREC_BF16X3)):
+            ops32 = (A.converted(np.float32, stream), B.converted(np.float32, stream))
+            stream.synchronize()
+            for name, pr in (("fp64", sht.PREC_FP64), ("fp32", sht.PREC_FP32), ("bf16x3", sht.PREC_BF16X3),
+                             ("fp32_f64in", sht.PREC_FP32), ("bf16x3_f64in", sht.PREC_BF16X3)):
                 C = sht.Tensor(ctx, Cc.block_sizes)
-                plan = sht.Plan(ctx, expr, A, B, C, pr)
+                Aop, Bop = ops32 if f32_operands(name) else (A, B)
+                plan = sht.Plan(ctx, expr, Aop, Bop, C, pr)
                 cands = plan.autotune_ex(1.0, 0.0, reps=3, stream=stream)
                 for _ in range(3):
                     plan.execute(1.0, 0.0, stream)
@@ -463,6 +473,7 @@ def leg_c3(sht, orc, stream):
                 fl = plan.stats()["flops"]
                 rec[name] = {"gflops": round(fl / med / 1e9, 3), "ms_p50": round(1e3 * med, 4),
                              "schedule": plan.get_schedule(), "variant": plan.get_variant(),
+                             "operands": "fp32" if f32_operands(name) else "fp64",
                              "parity": errors(C.download(), R, C.get_sparse_info())}
                 plan.close()
                 C.close()
@@ -578,10 +589,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200, help="c4: requests drawn for the stream")
     ap.add_argument("--serve-rates", default="50,200,1000", help="c4: Poisson arrival rates (requests/s)")
     ap.add_argument("--length", type=int, default=None, help="override every mode length (smaller runs)")
-    ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3",
+    ap.add_argument("--forms", default="fp64,fp64_exact,fp32,bf16x3,fp32_f64in,bf16x3_f64in",
                     help="fp64 (autotuned schedule), fp64_cs, fp64_as1, fp64_as2, fp64_as3, fp64_exact, fp32, "
                          "bf16x3; _cs = C-stationary (schedule 0), _asN = A-stationary schedule N (1: one-wave "
-                         "units, 2: four units of a block row per workgroup, 3: narrow sweeps)")
+                         "units, 2: four units of a block row per workgroup, 3: narrow sweeps); fp32 and bf16x3 "
+                         "read FP32 copies of A and B, fp32_f64in / bf16x3_f64in the FP64 tensors")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-flops", type=float, default=1.5e11, help="cap on the CPU sample's work")
@@ -611,7 +623,8 @@ def main():
     import shtensor_amd as sht
 
     t0 = time.time()
-    base = {"fp64": sht.PREC_FP64, "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3}
+    base = {"fp64": sht.PREC_FP64, "fp64_exact": sht.PREC_FP64_EXACT, "fp32": sht.PREC_FP32, "bf16x3": sht.PREC_BF16X3,
+            "fp32_f64in": sht.PREC_FP32, "bf16x3_f64in": sht.PREC_BF16X3}
     precs = dict(base)
     for b, pr in base.items():
         for suf in ("_cs", "_as1", "_as2", "_as3"):
@@ -626,6 +639,17 @@ def main():
     results = {}
     Ctens = {}
     plan_cold_ms = None
+    # The FP32 and split-BF16 forms read FP32 operand tensors (A and B rounded
+    # once, on the device, outside the timed steps; the one-time cost is
+    # reported as config.operand_convert_ms); *_f64in reads the FP64 tensors.
+    ops32, convert_ms = None, None
+    if any(f32_operands(f) for f in forms):
+        stream.synchronize()
+        tc = time.perf_counter()
+        ops32 = (A.converted(np.float32, stream), B.converted(np.float32, stream))
+        stream.synchronize()
+        convert_ms = round(1e3 * (time.perf_counter() - tc), 2)
+        log("FP32 operand copies of A and B in %.1f ms (incl. reserving them)" % convert_ms)
     if forms:
         # The first plan of a process also loads the planner's code objects
         # and sizes its sort buffers; it is timed apart (plan_cold_ms) so that
@@ -638,8 +662,9 @@ def main():
         wc.close()
     for f in forms:
         Cf = sht.Tensor(ctx, Cc.block_sizes)
+        Aop, Bop = ops32 if f32_operands(f) else (A, B)
         tp = time.perf_counter()
-        plan = sht.Plan(ctx, expr, A, B, Cf, precs[f])
+        plan = sht.Plan(ctx, expr, Aop, Bop, Cf, precs[f])
         plan_ms = 1e3 * (time.perf_counter() - tp)  # task list, C pattern, default schedule: no autotuning
         build = plan.build_ms()
         cands = None
@@ -827,6 +852,7 @@ def main():
                         "schedule": stf.get("schedule"), "variant": stf.get("variant"),
                         "autotune_ms": stf.get("autotune_ms"),
                         "autotuned_vs_fixed_tile": stf.get("autotuned_vs_fixed_tile"),
+                        "operands": "fp32" if f32_operands(f) else "fp64",
                         "graph_replays": None if results[f]["graph"] is None else {
                             "n": len(results[f]["graph"]),
                             "ms_p50": round(1e3 * float(np.percentile(results[f]["graph"], 50)), 4),
@@ -848,7 +874,8 @@ def main():
         "config": {"workload": "%s %s L=%d" % (args.config, expr, args.length or CONFIGS[args.config][1]),
                    "ntasks": r["stats"]["ntasks"], "ncblocks": r["stats"]["ncblocks"],
                    "flops_per_step": r["stats"]["flops"], "graph": bool(args.graph),
-                   "plan_ms": r["stats"]["plan_ms"], "plan_cold_ms": plan_cold_ms},
+                   "plan_ms": r["stats"]["plan_ms"], "plan_cold_ms": plan_cold_ms,
+                   "operand_convert_ms": convert_ms},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(r["times"], 50)), 4),
                        "p99": round(1e3 * float(np.percentile(r["times"], 99)), 4)},
         "forms": forms_out,

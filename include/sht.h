@@ -155,6 +155,12 @@ sht_status sht_tensor_occupation(const sht_tensor* t, double* occ);
 sht_status sht_tensor_upload(sht_tensor* t, const void* host, int64_t nelem);
 sht_status sht_tensor_download(const sht_tensor* t, void* host, int64_t nelem);
 /* Per-block norms into a device array of nnz doubles (FP64 accumulation).  */
+/* No counterpart in the reference (Tensor<T,N> fixes T per tensor): dst (same
+ * block sizes, same context; unreserved, or reserved with src's pattern)
+ * takes src's block pattern and src's values converted to dst's float type
+ * (FP64 -> FP32 rounds to nearest even).  Asynchronous on `stream` after the
+ * reservation.  The FP32 and BF16X3 contraction forms read FLOAT32 A and B. */
+sht_status sht_tensor_convert(const sht_tensor* src, sht_tensor* dst, void* stream);
 sht_status sht_tensor_block_norms(const sht_tensor* t, sht_block_norm norm, double* d_out, void* stream);
 /* Tensor::filter (Tensor.h:336-363): block ids with norm < eps are negated
  * (-id-1); compress!=0 then compacts (BaseTensor::compress, fixed).        */
@@ -174,7 +180,13 @@ sht_status sht_gen_fill(sht_tensor* t, uint64_t seed, int tid, double lambda, vo
  * C = beta*C + alpha * contract(expr, A, B).  If C holds no blocks its
  * pattern is built from the product (sorted unique product blocks) and
  * reserved; otherwise C's pattern is kept and products outside it dropped.
- * The task list and pattern are built on the GPU.                           */
+ * The task list and pattern are built on the GPU.
+ * Tensor types: C is FLOAT64 in every form (the sum over contracted blocks
+ * runs in FP64).  A and B share one type: FLOAT64 for every form, or
+ * FLOAT32 for SHT_PREC_FP32 / SHT_PREC_BF16X3 (both round each operand to
+ * FP32 first, so FLOAT32 operands give the same C bit for bit as the FLOAT64
+ * tensors they were rounded from, with half the operand bytes).  Other
+ * combinations fail with SHT_ERR_UNSUPPORTED.                              */
 sht_status sht_contract(sht_ctx* ctx, const char* expr, double alpha, const sht_tensor* A, const sht_tensor* B,
                         double beta, sht_tensor* C, sht_precision prec, void* stream);
 

@@ -318,6 +318,13 @@ class Tensor:
         call("sht_device_synchronize")
         return out.numpy()[:n]
 
+    def converted(self, dtype, stream=None):
+        """New tensor of the same block pattern with this tensor's values in
+        `dtype` (sht_tensor_convert; FP64 -> FP32 rounds to nearest)."""
+        out = Tensor(self.ctx, self.block_sizes, dtype=dtype)
+        call("sht_tensor_convert", C.c_void_p(self.ptr), C.c_void_p(out.ptr), _sptr(stream))
+        return out
+
     def filter(self, eps, norm=FROBENIUS, compress=True):
         call("sht_tensor_filter", C.c_void_p(self.ptr), float(eps), norm, 1 if compress else 0)
 

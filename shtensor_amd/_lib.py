@@ -68,6 +68,7 @@ SIGNATURES = {
     "sht_tensor_occupation": [_P, C.POINTER(_D)],
     "sht_tensor_upload": [_P, _P, _I64],
     "sht_tensor_download": [_P, _P, _I64],
+    "sht_tensor_convert": [_P, _P, _P],
     "sht_tensor_block_norms": [_P, _I, _P, _P],
     "sht_tensor_filter": [_P, _D, _I, _I],
     "sht_tensor_compress": [_P],

@@ -126,16 +126,17 @@ __device__ inline const double* c_src(const AsArgs& p, int64_t e, int row)
 // once per unit: non-temporal loads keep the stream from evicting the panels
 // and C tiles that the other units of the block row reuse in L2.
 // ---------------------------------------------------------------------------
+template <class TE>
 struct AStream {
-  const double* a;  // A block (wave-uniform)
+  const TE* a;  // A block (wave-uniform)
   int K, M;
 };
 
 // A fragments of step st of the stream (rows row + 16t, column 4st + lk).  A
 // blocks of the schedule hold fewer than 2^28 elements, so the element
 // offset fits 32 bits: the load is a uniform base plus a lane offset.
-template <int R>
-__device__ inline void load_a(const AStream& s, int st, int row, int lk, double (&a)[R])
+template <int R, class TE>
+__device__ inline void load_a(const AStream<TE>& s, int st, int row, int lk, TE (&a)[R])
 {
   const uint32_t off = static_cast<uint32_t>(min(4 * st + lk, s.K - 1) * s.M + row);
 #pragma unroll
@@ -144,12 +145,12 @@ __device__ inline void load_a(const AStream& s, int st, int row, int lk, double 
 
 // Step st of the current sweep's stream when st < nsteps, else step
 // st - nsteps of the next sweep's (the selects are wave-uniform).
-template <int R>
-__device__ inline void load_a2(const AStream& cu, const AStream& nx, int nsteps, int st, int row, int lk,
-                               double (&a)[R])
+template <int R, class TE>
+__device__ inline void load_a2(const AStream<TE>& cu, const AStream<TE>& nx, int nsteps, int st, int row, int lk,
+                               TE (&a)[R])
 {
   const bool n = st >= nsteps;
-  AStream s;
+  AStream<TE> s;
   s.a = n ? nx.a : cu.a;
   s.K = n ? nx.K : cu.K;
   s.M = n ? nx.M : cu.M;
@@ -163,6 +164,27 @@ __device__ inline __attribute__((unused)) void load_a_pair(const double* a, uint
   const v2d8 v = __builtin_nontemporal_load(reinterpret_cast<const v2d8*>(a + off));
   x[0] = v.x;
   x[1] = v.y;
+}
+// FP32 operands (FP32 and split-BF16 forms): the same two rows, 8 bytes.
+typedef float v2f4 __attribute__((ext_vector_type(2), aligned(4)));
+__device__ inline __attribute__((unused)) void load_a_pair(const float* a, uint32_t off, float (&x)[2])
+{
+  const v2f4 v = __builtin_nontemporal_load(reinterpret_cast<const v2f4*>(a + off));
+  x[0] = v.x;
+  x[1] = v.y;
+}
+// Two consecutive elements of an operand (plain load), widened to double.
+__device__ inline __attribute__((unused)) void load_pair(const double* a, double& x, double& y)
+{
+  const v2d8 v = *reinterpret_cast<const v2d8*>(a);
+  x = v.x;
+  y = v.y;
+}
+__device__ inline __attribute__((unused)) void load_pair(const float* a, float& x, float& y)
+{
+  const v2f4 v = *reinterpret_cast<const v2f4*>(a);
+  x = v.x;
+  y = v.y;
 }
 
 template <int F> struct BElem { typedef double type; };
@@ -179,10 +201,11 @@ template <> struct BElem<1> { typedef float type; };
 // the consumer after the scheduling barrier.  `cbs` (LDS) holds the C offset
 // of each sweep column.
 // ---------------------------------------------------------------------------
-template <int R, int NS, int F, bool A1>
+template <int R, int NS, int F, bool A1, class TE>
 __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const SweepU& nxs, int r0, int li, int lk,
-                                  double (&ar)[4][R], const int64_t* cbs)
+                                  TE (&ar)[4][R], const int64_t* cbs)
 {
+  static_assert(F != 0 || std::is_same<TE, double>::value, "FP64 MFMA reads FP64 operands");
   typedef typename BElem<F>::type TB;
   typedef typename std::conditional<F == 0, v4d, v4f>::type Acc;
   const TB* Bl = static_cast<const TB*>(p.B) + sw.b_off + li + static_cast<int64_t>(lk) * sw.pitch;
@@ -220,18 +243,18 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
   const int nsteps = ((K + 15) >> 4) * 4, last = nsteps - 1;
   loadB(0, b0);
   __builtin_amdgcn_sched_barrier(0);
-  AStream cu, nx;
-  cu.a = static_cast<const double*>(p.A) + sw.a_off;
+  AStream<TE> cu, nx;
+  cu.a = static_cast<const TE*>(p.A) + sw.a_off;
   cu.K = K;
   cu.M = M;
-  nx.a = static_cast<const double*>(p.A) + nxs.a_off;
+  nx.a = static_cast<const TE*>(p.A) + nxs.a_off;
   nx.K = nxs.k;
   nx.M = nxs.m;
   // steps past the contracted length (the loop runs whole 16-k iterations)
   // issue no MFMAs: every product they hold is zero
   const int kq = (K + 3) >> 2;
   const double apad = pad_a(p.alpha, A1);
-  auto mma = [&](int st, const double(&araw)[R], const TB(&b)[NS]) {
+  auto mma = [&](int st, const TE(&araw)[R], const TB(&b)[NS]) {
     if (st >= kq) return;
     const bool kin = 4 * st + lk < K;
 #pragma unroll
@@ -333,9 +356,9 @@ __device__ inline void sweep_mfma(const AsArgs& p, const SweepU& sw, const Sweep
 // ---------------------------------------------------------------------------
 constexpr int kAslStages = 4;
 
-template <int NS, int F, bool A1, int W, int AD>
+template <int NS, int F, bool A1, int W, int AD, class TE>
 __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU& nxs, int r0, int li, int lk, int wave,
-                                 bool ghost, double (&ar)[AD][2], const int64_t* cbs,
+                                 bool ghost, TE (&ar)[AD][2], const int64_t* cbs,
                                  typename BElem<F>::type (*stage)[kAsSweepSlots][64])
 {
   constexpr int R = 2;
@@ -409,7 +432,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // (offsets grow with the column, so clamping the offset is the same), and
   // at step nstepsU the stream moves on to the next sweep's A block.  Ghost
   // waves use K = 1 (one fixed line).
-  const double* abase = static_cast<const double*>(p.A) + sw.a_off;
+  const TE* abase = static_cast<const TE*>(p.A) + sw.a_off;
   // (a lane whose two rows are both past the block's last row re-reads rows
   // 0 and 1 instead of running on into the next columns: its products only
   // reach padding rows of the C tiles; a lane holding the last real row and
@@ -419,7 +442,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   uint32_t arun = static_cast<uint32_t>((4 * (AD - 1) + lk) * M + ra2);
   uint32_t alast = static_cast<uint32_t>((ghost ? 0 : K - 1) * M + ra2);
   uint32_t a4m = static_cast<uint32_t>(4 * M);
-  const double* nbase = static_cast<const double*>(p.A) + nxs.a_off;
+  const TE* nbase = static_cast<const TE*>(p.A) + nxs.a_off;
   const uint32_t nrun = static_cast<uint32_t>(lk * nxs.m + ra2);
   const uint32_t nlast = static_cast<uint32_t>((ghost ? 0 : nxs.k - 1) * nxs.m + ra2);
   const uint32_t n4m = static_cast<uint32_t>(4 * nxs.m);
@@ -427,7 +450,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
   // one step's MFMAs (the A values already masked where the step passes K)
   const int kfull = K >> 2;  // steps whose four contracted indices are < K
   const double apad = pad_a(p.alpha, A1);
-  auto mma_full = [&](const double(&araw)[R]) {
+  auto mma_full = [&](const TE(&araw)[R]) {
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if constexpr (F == 0) {
@@ -468,7 +491,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
       // that step (the empty asm statement keeps the branch from being
       // turned back into per-step selects), and every step runs the same
       // MFMA sequence.
-      double am[R];
+      TE am[R];
 #pragma unroll
       for (int t = 0; t < R; ++t) am[t] = ar[j][t];
       if (st + j >= kfull) {
@@ -476,7 +499,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 #pragma unroll
         for (int t = 0; t < R; ++t) {
           asm volatile("" : "+v"(am[t]));
-          am[t] = kin ? am[t] : (F == 0 ? apad : 0.0);
+          am[t] = kin ? am[t] : static_cast<TE>(F == 0 ? apad : 0.0);
         }
       }
       mma_full(am);
@@ -555,7 +578,7 @@ __device__ inline void sweep_lds(const AsArgs& p, const SweepU& sw, const SweepU
 
 // One workgroup task of the grouped kernel: the W units (row chunks) of one
 // block row walk the row's sweep list.
-template <int F, int W, bool A1, int AD>
+template <int F, int W, bool A1, int AD, class TE>
 __device__ __attribute__((always_inline)) inline void asl_task(const AsArgs& p, int wave, int lane, int li, int lk,
                                                                int nsw, int s0, int r0, bool ghost, int64_t* cbs,
                                                                typename BElem<F>::type (*stage)[kAsSweepSlots][64])
@@ -565,10 +588,10 @@ __device__ __attribute__((always_inline)) inline void asl_task(const AsArgs& p, 
   AsSweep raw2 = p.sweeps[s0 + min(2, nsw - 1)];
   int64_t cc0, cc1;
   load_cols(p, cur, lane, cc0, cc1);
-  double ar[AD][2];
+  TE ar[AD][2];
   {
     // first steps of the A stream (rows r0 + 2li, +1; ghosts: one fixed line)
-    const double* a = static_cast<const double*>(p.A) + cur.a_off;
+    const TE* a = static_cast<const TE*>(p.A) + cur.a_off;
     const int kmax = ghost ? 0 : cur.k - 1;
 #pragma unroll
     for (int i = 0; i < AD - 1; ++i)
@@ -581,14 +604,14 @@ __device__ __attribute__((always_inline)) inline void asl_task(const AsArgs& p, 
     wave_sync();
     load_cols(p, nxt, lane, cc0, cc1);
     switch (cur.ns) {
-      case 1: sweep_lds<1, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      case 2: sweep_lds<2, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      case 3: sweep_lds<3, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      case 4: sweep_lds<4, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      case 5: sweep_lds<5, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      case 6: sweep_lds<6, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      case 7: sweep_lds<7, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
-      default: sweep_lds<8, F, A1, W, AD>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 1: sweep_lds<1, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 2: sweep_lds<2, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 3: sweep_lds<3, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 4: sweep_lds<4, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 5: sweep_lds<5, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 6: sweep_lds<6, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      case 7: sweep_lds<7, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
+      default: sweep_lds<8, F, A1, W, AD, TE>(p, cur, nxt, r0, li, lk, wave, ghost, ar, cbs, stage); break;
     }
     wave_sync();
     cur = nxt;
@@ -606,7 +629,7 @@ __device__ __attribute__((always_inline)) inline void asl_task(const AsArgs& p, 
 // next task from that queue's counter until the queue is empty.  The task
 // index is read by every wave after a barrier, so every exit and skip is
 // taken by the whole workgroup.
-template <int F, int W, bool A1, int AD>
+template <int F, int W, bool A1, int AD, class TE>
 __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
 {
   static_assert(AD == 4 || AD == 8, "A ring of 4 or 8 steps");
@@ -631,7 +654,7 @@ __global__ __launch_bounds__(64 * W) void k_asl(AsArgs p)
     const int nsw = uni(u.nsweep), s0 = uni(u.sweep0), r0 = uni(u.r0);
     const bool ghost = uni(u.pad) != 0;
     if (nsw == 0) continue;  // a whole-workgroup filler (every wave of it)
-    asl_task<F, W, A1, AD>(p, wave, lane, li, lk, nsw, s0, r0, ghost, cbs, stage);
+    asl_task<F, W, A1, AD, TE>(p, wave, lane, li, lk, nsw, s0, r0, ghost, cbs, stage);
   }
 }
 
@@ -647,9 +670,8 @@ __device__ inline unsigned short bf16_bits(float x)
   return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-__device__ inline void split3(double xd, unsigned short& h, unsigned short& m, unsigned short& l)
+__device__ inline void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l)
 {
-  const float x = static_cast<float>(xd);
   h = bf16_bits(x);
   const float r1 = x - __uint_as_float(static_cast<unsigned int>(h) << 16);
   m = bf16_bits(r1);
@@ -657,10 +679,10 @@ __device__ inline void split3(double xd, unsigned short& h, unsigned short& m, u
   l = bf16_bits(r2);
 }
 
-template <int R, int NS>
+template <int R, int NS, class TE>
 __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int li, int lk, const int64_t* cbs)
 {
-  const double* A = static_cast<const double*>(p.A) + sw.a_off;
+  const TE* A = static_cast<const TE*>(p.A) + sw.a_off;
   const int kp = sw.pitch;
   const unsigned short* Bl =
       static_cast<const unsigned short*>(p.B) + sw.b_off + static_cast<int64_t>(li) * kp + 8 * lk;
@@ -675,7 +697,7 @@ __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int 
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
   const int nsteps = (K + kBfK - 1) / kBfK;
-  double x_nx[R][8];
+  TE x_nx[R][8];
   auto load_a8 = [&](int st) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -683,8 +705,8 @@ __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int 
       const uint32_t off = static_cast<uint32_t>(min(kk, K - 1) * M + rl);
 #pragma unroll
       for (int t = 0; t < R; ++t) {
-        const double x = A[off + 16u * t];
-        x_nx[t][e] = kk < K ? x : 0.0;
+        const TE x = A[off + 16u * t];
+        x_nx[t][e] = kk < K ? x : static_cast<TE>(0);
       }
     }
   };
@@ -696,7 +718,7 @@ __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int 
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         unsigned short h, m, l;
-        split3(x_nx[t][e], h, m, l);
+        split3(static_cast<float>(x_nx[t][e]), h, m, l);
         ah[t][e] = static_cast<short>(h);
         am[t][e] = static_cast<short>(m);
         al[t][e] = static_cast<short>(l);
@@ -745,12 +767,12 @@ __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int 
   }
 }
 
-template <int R, int NS, int F, bool A1>
+template <int R, int NS, int F, bool A1, class TE>
 __device__ inline void run_sweep(const AsArgs& p, const SweepU& sw, const SweepU& nx, int r0, int li, int lk,
-                                 double (&ar)[4][R], const int64_t* cbs)
+                                 TE (&ar)[4][R], const int64_t* cbs)
 {
-  if constexpr (F == 2) sweep_bf3<R, NS>(p, sw, r0, li, lk, cbs);
-  else sweep_mfma<R, NS, F, A1>(p, sw, nx, r0, li, lk, ar, cbs);
+  if constexpr (F == 2) sweep_bf3<R, NS, TE>(p, sw, r0, li, lk, cbs);
+  else sweep_mfma<R, NS, F, A1, TE>(p, sw, nx, r0, li, lk, ar, cbs);
 }
 
 // One wave per unit of 16*R rows, WPG waves (units of one block row) per
@@ -766,7 +788,7 @@ __device__ inline void run_sweep(const AsArgs& p, const SweepU& sw, const SweepU
 #else
 #define SHT_ASP_OCC
 #endif
-template <int R, int NSMAX, int F, int WPG, bool A1>
+template <int R, int NSMAX, int F, int WPG, bool A1, class TE>
 __global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
 {
   static_assert(16 * NSMAX <= 128, "two column offsets per lane");
@@ -784,10 +806,10 @@ __global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
   AsSweep raw2 = p.sweeps[s0 + min(2, nsw - 1)];
   int64_t cc0, cc1;
   load_cols(p, cur, lane, cc0, cc1);
-  double ar[4][R];
+  TE ar[4][R];
   if constexpr (F != 2) {
-    AStream s;
-    s.a = static_cast<const double*>(p.A) + cur.a_off;
+    AStream<TE> s;
+    s.a = static_cast<const TE*>(p.A) + cur.a_off;
     s.K = cur.k;
     s.M = cur.m;
     load_a(s, 0, r0 + li, lk, ar[0]);
@@ -800,14 +822,14 @@ __global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
     wave_sync();
     load_cols(p, nxt, lane, cc0, cc1);
     switch (min(cur.ns, NSMAX)) {
-      case 1: run_sweep<R, 1, F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      case 2: run_sweep<R, cmin(2, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      case 3: run_sweep<R, cmin(3, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      case 4: run_sweep<R, cmin(4, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      case 5: run_sweep<R, cmin(5, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      case 6: run_sweep<R, cmin(6, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      case 7: run_sweep<R, cmin(7, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
-      default: run_sweep<R, cmin(8, NSMAX), F, A1>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 1: run_sweep<R, 1, F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 2: run_sweep<R, cmin(2, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 3: run_sweep<R, cmin(3, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 4: run_sweep<R, cmin(4, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 5: run_sweep<R, cmin(5, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 6: run_sweep<R, cmin(6, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      case 7: run_sweep<R, cmin(7, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
+      default: run_sweep<R, cmin(8, NSMAX), F, A1, TE>(p, cur, nxt, r0, li, lk, ar, cbs); break;
     }
     wave_sync();  // the column offsets are rewritten by the next sweep
     cur = nxt;
@@ -830,14 +852,14 @@ __global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
 // ---------------------------------------------------------------------------
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
-template <int NS>
+template <int NS, class TE>
 __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, int li, int lk, int wave, bool ghost,
                                      const int64_t* cbs, v4u (*stage)[3][kAsSweepSlots][64])
 {
   constexpr int R = 2;
   constexpr int NSW = (NS + 3) / 4;
   const int lane = li + 16 * lk;
-  const double* A = static_cast<const double*>(p.A) + sw.a_off;
+  const TE* A = static_cast<const TE*>(p.A) + sw.a_off;
   const int kp = sw.pitch;
   const unsigned short* Bl =
       static_cast<const unsigned short*>(p.B) + sw.b_off + static_cast<int64_t>(li) * kp + 8 * lk;
@@ -876,17 +898,15 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
       for (int pl = 0; pl < 3; ++pl) stage[stg][pl][wave + 4 * x][lane] = g[x][pl];
   };
   // raw A of step st: rows rl2, rl2 + 1, contracted indices 32st + 8lk + e
-  auto loadA = [&](int st, double (&x)[R][8]) {
+  auto loadA = [&](int st, TE (&x)[R][8]) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int kk = kBfK * min(st, nsteps - 1) + 8 * lk + e;
       const uint32_t off = ghost ? static_cast<uint32_t>(ra2) : static_cast<uint32_t>(min(kk, K - 1) * M + ra2);
-      const v2d8 v = *reinterpret_cast<const v2d8*>(A + off);
-      x[0][e] = v.x;
-      x[1][e] = v.y;
+      load_pair(A + off, x[0][e], x[1][e]);
     }
   };
-  auto step = [&](int st, const double (&x)[R][8], int stg) {
+  auto step = [&](int st, const TE (&x)[R][8], int stg) {
     v8s ah[R], am[R], al[R];
 #pragma unroll
     for (int t = 0; t < R; ++t)
@@ -894,7 +914,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
       for (int e = 0; e < 8; ++e) {
         const int kk = kBfK * st + 8 * lk + e;
         unsigned short h, m, l;
-        split3(kk < K ? x[t][e] : 0.0, h, m, l);
+        split3(kk < K ? static_cast<float>(x[t][e]) : 0.f, h, m, l);
         ah[t][e] = static_cast<short>(h);
         am[t][e] = static_cast<short>(m);
         al[t][e] = static_cast<short>(l);
@@ -915,7 +935,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
       }
     }
   };
-  double x0[R][8], x1[R][8];
+  TE x0[R][8], x1[R][8];
   loadP(0);
   writeP(0);
   loadA(0, x0);
@@ -968,6 +988,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
 // Workgroups of four waves over four 32-row chunks of one block row (the
 // LDS-shared split-BF16 path).  Every wave runs the same sweep list, ghosts
 // included, so the workgroup barriers match.
+template <class TE>
 __global__ __launch_bounds__(256) SHT_ASP_OCC void k_asb(AsArgs p)
 {
   constexpr int W = 4;
@@ -993,14 +1014,14 @@ __global__ __launch_bounds__(256) SHT_ASP_OCC void k_asb(AsArgs p)
     wave_sync();
     load_cols(p, nxt, lane, cc0, cc1);
     switch (cur.ns) {
-      case 1: sweep_bf3_lds<1>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      case 2: sweep_bf3_lds<2>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      case 3: sweep_bf3_lds<3>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      case 4: sweep_bf3_lds<4>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      case 5: sweep_bf3_lds<5>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      case 6: sweep_bf3_lds<6>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      case 7: sweep_bf3_lds<7>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
-      default: sweep_bf3_lds<8>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 1: sweep_bf3_lds<1, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 2: sweep_bf3_lds<2, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 3: sweep_bf3_lds<3, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 4: sweep_bf3_lds<4, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 5: sweep_bf3_lds<5, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 6: sweep_bf3_lds<6, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      case 7: sweep_bf3_lds<7, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
+      default: sweep_bf3_lds<8, TE>(p, cur, r0, li, lk, wave, ghost, cbs, stage); break;
     }
     wave_sync();
     cur = nxt;
@@ -1071,14 +1092,14 @@ __global__ __launch_bounds__(64) void k_as_exact(AsArgs p)
 // ---------------------------------------------------------------------------
 // Panel conversion, once per execution (one workgroup per B block record).
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_bt(const BtRec* recs, int64_t nrec, const double* B, T* BT)
+template <class T, class TE>
+__global__ __launch_bounds__(256) void k_bt(const BtRec* recs, int64_t nrec, const TE* B, T* BT)
 {
   for (int64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
     const BtRec rec = recs[r];
     const int krows = (rec.k + 15) / 16 * 16;  // zero rows up to a multiple of 16
     const int64_t tot = static_cast<int64_t>(krows) * rec.n;
-    const double* b = B + rec.b_off;
+    const TE* b = B + rec.b_off;
     T* o = BT + rec.bt_off;
     for (int64_t x = threadIdx.x; x < tot; x += blockDim.x) {
       const int k = static_cast<int>(x / rec.n), j = static_cast<int>(x - static_cast<int64_t>(k) * rec.n);
@@ -1088,19 +1109,20 @@ __global__ __launch_bounds__(256) void k_bt(const BtRec* recs, int64_t nrec, con
   }
 }
 
-__global__ __launch_bounds__(256) void k_bt_bf3(const BtRec* recs, int64_t nrec, const double* B, unsigned short* BT,
+template <class TE>
+__global__ __launch_bounds__(256) void k_bt_bf3(const BtRec* recs, int64_t nrec, const TE* B, unsigned short* BT,
                                                 int64_t plane)
 {
   for (int64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
     const BtRec rec = recs[r];
     const int kp = rec.pitch;
     const int64_t tot = static_cast<int64_t>(kp) * rec.n;
-    const double* b = B + rec.b_off;
+    const TE* b = B + rec.b_off;
     unsigned short* o = BT + rec.bt_off;
     for (int64_t x = threadIdx.x; x < tot; x += blockDim.x) {
       const int j = static_cast<int>(x / kp), k = static_cast<int>(x - static_cast<int64_t>(j) * kp);
       unsigned short h = 0, m = 0, l = 0;
-      if (k < rec.k) split3(b[k + static_cast<int64_t>(j) * rec.k], h, m, l);
+      if (k < rec.k) split3(static_cast<float>(b[k + static_cast<int64_t>(j) * rec.k]), h, m, l);
       o[x] = h;
       o[x + plane] = m;
       o[x + 2 * plane] = l;
@@ -1131,8 +1153,14 @@ void launch_as_bf3(AsClass k, const AsArgs& a, hipStream_t s)
   const dim3 g1(static_cast<unsigned>(a.nunits)), b1(64);
   const dim3 gg(static_cast<unsigned>((a.nunits + G - 1) / G)), bg(64 * G);
   switch (k) {
-    case AsClass::Bf16x3: hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Bf16x3G: hipLaunchKernelGGL(k_asb, gg, bg, 0, s, a); break;
+    case AsClass::Bf16x3:
+      if (a.f32) hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false, float>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false, double>), g1, b1, 0, s, a);
+      break;
+    case AsClass::Bf16x3G:
+      if (a.f32) hipLaunchKernelGGL(k_asb<float>, gg, bg, 0, s, a);
+      else hipLaunchKernelGGL(k_asb<double>, gg, bg, 0, s, a);
+      break;
     default: throw Error(SHT_ERR_INVALID, "not a split-BF16 class");
   }
 }
@@ -1199,47 +1227,69 @@ void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
   };
   static_assert(kAsUnitRows == 32 && kAsUnitRowsW == 64, "unit rows: two MFMA tiles; 64 lanes");
   static_assert(kAsSweepSlots == 8, "sweep slots");
+  SHT_REQUIRE(!a.f32 || k == AsClass::Mfma32 || k == AsClass::Mfma32G || k == AsClass::Mfma32N ||
+                  k == AsClass::Bf16x3 || k == AsClass::Bf16x3G,
+              SHT_ERR_STATE, "FP32 operands reach an FP64 kernel class");
   // the grouped classes read W unit records per workgroup
   SHT_REQUIRE(as_group_waves(k) == 1 || a.nunits % G == 0, SHT_ERR_STATE, "grouped unit list not padded");
   switch (k) {
     case AsClass::Mfma64:
-      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, 8, 0, 1, true>), g1, b1, 0, s, a);
-      else hipLaunchKernelGGL((k_asp<2, 8, 0, 1, false>), g1, b1, 0, s, a);
+      if (a.alpha_one) hipLaunchKernelGGL((k_asp<2, 8, 0, 1, true, double>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<2, 8, 0, 1, false, double>), g1, b1, 0, s, a);
       break;
     case AsClass::Mfma64G:
-      if (a.alpha_one) persistent(k_asl<0, G, true, 4>);
-      else persistent(k_asl<0, G, false, 4>);
+      if (a.alpha_one) persistent(k_asl<0, G, true, 4, double>);
+      else persistent(k_asl<0, G, false, 4, double>);
       break;
-    case AsClass::Mfma32: hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false>), g1, b1, 0, s, a); break;
-    case AsClass::Mfma32G: persistent(k_asl<1, G, false, 4>); break;
+    case AsClass::Mfma32:
+      if (a.f32) hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false, float>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<2, 8, 1, 1, false, double>), g1, b1, 0, s, a);
+      break;
+    case AsClass::Mfma32G:
+      if (a.f32) persistent(k_asl<1, G, false, 4, float>);
+      else persistent(k_asl<1, G, false, 4, double>);
+      break;
     case AsClass::Bf16x3:
     case AsClass::Bf16x3G: launch_as_bf3(k, a, s); break;
     case AsClass::Mfma64N:
-      if (a.alpha_one) hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 0, 1, true>), g1, b1, 0, s, a);
-      else hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 0, 1, false>), g1, b1, 0, s, a);
+      if (a.alpha_one) hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 0, 1, true, double>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 0, 1, false, double>), g1, b1, 0, s, a);
       break;
-    case AsClass::Mfma32N: hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 1, 1, false>), g1, b1, 0, s, a); break;
+    case AsClass::Mfma32N:
+      if (a.f32) hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 1, 1, false, float>), g1, b1, 0, s, a);
+      else hipLaunchKernelGGL((k_asp<4, kAsNarrowSlots, 1, 1, false, double>), g1, b1, 0, s, a);
+      break;
     case AsClass::Exact: hipLaunchKernelGGL(k_as_exact, g1, b1, 0, s, a); break;
     default: throw Error(SHT_ERR_INVALID, "unknown A-stationary class");
   }
   SHT_HIP(hipGetLastError());
 }
 
-void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const double* B, void* BT, int64_t bt_total,
+void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const void* B, bool b_f32, void* BT, int64_t bt_total,
                hipStream_t s)
 {
   if (nrec <= 0) return;
   const int g = static_cast<int>(nrec < 65536 ? nrec : 65536);
+  const double* b64 = static_cast<const double*>(B);
+  const float* b32 = static_cast<const float*>(B);
+  SHT_REQUIRE(!b_f32 || f != BtFormat::F64T, SHT_ERR_STATE, "FP32 operand for an FP64 panel");
   switch (f) {
     case BtFormat::F64T:
-      hipLaunchKernelGGL(k_bt<double>, dim3(g), dim3(256), 0, s, recs, nrec, B, static_cast<double*>(BT));
+      hipLaunchKernelGGL((k_bt<double, double>), dim3(g), dim3(256), 0, s, recs, nrec, b64, static_cast<double*>(BT));
       break;
     case BtFormat::F32T:
-      hipLaunchKernelGGL(k_bt<float>, dim3(g), dim3(256), 0, s, recs, nrec, B, static_cast<float*>(BT));
+      if (b_f32)
+        hipLaunchKernelGGL((k_bt<float, float>), dim3(g), dim3(256), 0, s, recs, nrec, b32, static_cast<float*>(BT));
+      else
+        hipLaunchKernelGGL((k_bt<float, double>), dim3(g), dim3(256), 0, s, recs, nrec, b64, static_cast<float*>(BT));
       break;
     case BtFormat::BF16X3:
-      hipLaunchKernelGGL(k_bt_bf3, dim3(g), dim3(256), 0, s, recs, nrec, B, static_cast<unsigned short*>(BT),
-                         bt_total);
+      if (b_f32)
+        hipLaunchKernelGGL(k_bt_bf3<float>, dim3(g), dim3(256), 0, s, recs, nrec, b32,
+                           static_cast<unsigned short*>(BT), bt_total);
+      else
+        hipLaunchKernelGGL(k_bt_bf3<double>, dim3(g), dim3(256), 0, s, recs, nrec, b64,
+                           static_cast<unsigned short*>(BT), bt_total);
       break;
   }
   SHT_HIP(hipGetLastError());

@@ -38,6 +38,23 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 typedef double v2d8 __attribute__((ext_vector_type(2), aligned(8)));  // 16 bytes, 8-byte aligned
+typedef float v2f4 __attribute__((ext_vector_type(2), aligned(4)));   // 8 bytes, 4-byte aligned
+
+// Two consecutive operand elements, widened to double (FP64 tensors, or the
+// FP32 tensors the FP32 and split-BF16 forms may read; the widening is
+// exact, so the kernels' later rounding to FP32 or bf16 sees the stored value).
+__device__ inline void ld2(const double* p, double& x, double& y)
+{
+  const v2d8 v = *reinterpret_cast<const v2d8*>(p);
+  x = v.x;
+  y = v.y;
+}
+__device__ inline void ld2(const float* p, double& x, double& y)
+{
+  const v2f4 v = *reinterpret_cast<const v2f4*>(p);
+  x = v.x;
+  y = v.y;
+}
 
 namespace {
 
@@ -239,9 +256,10 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 // after task in task order and k ascending, so every C element sees the
 // same chain as in the one-wave kernels of the same precision.
 // ---------------------------------------------------------------------------
-template <int F, int TR, int TC>
+template <int F, int TR, int TC, class TE = double>
 __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 {
+  static_assert(F != 0 || std::is_same<TE, double>::value, "FP64 MFMA reads FP64 operands");
   // waves: WR along the rows x WC along the columns of the tile
   constexpr int WC = TC >= 32 ? 2 : 1, WR = 4 / WC;
   constexpr int TRt = TR / WR / 16, TCt = TC / WC / 16;  // 16 x 16 MFMA tiles per wave
@@ -255,8 +273,8 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int wr = (TR / WR) * (w % WR), wc = (TC / WC) * (w / WR);
-  const double* A = static_cast<const double*>(p.A);
-  const double* B = static_cast<const double*>(p.B);
+  const TE* A = static_cast<const TE*>(p.A);
+  const TE* B = static_cast<const TE*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
   // Row padding of the LDS stages against bank conflicts of the loaders'
   // stores and the waves' fragment reads, per tile shape (measured on C3
@@ -307,22 +325,14 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
     const int K = tr.k;                                                                    \
     const int kk = kl + kA;                                                                \
     ka = kk < K;                                                                           \
-    const double* pa = A + tr.a_off + static_cast<int64_t>(min(kk, K - 1)) * M + ia;       \
-    _Pragma("unroll") for (int e = 0; e < EA; e += 2) {                                    \
-      const v2d8 x = *reinterpret_cast<const v2d8*>(pa + e);                               \
-      ra[e] = x.x;                                                                         \
-      ra[e + 1] = x.y;                                                                     \
-    }                                                                                      \
+    const TE* pa = A + tr.a_off + static_cast<int64_t>(min(kk, K - 1)) * M + ia;           \
+    _Pragma("unroll") for (int e = 0; e < EA; e += 2) ld2(pa + e, ra[e], ra[e + 1]);       \
     kbb = kl + kB;                                                                         \
     Kb = K;                                                                                \
-    const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(kbb, K - 1);      \
+    const TE* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(kbb, K - 1);          \
     if constexpr (EB == 1) rb[0] = pb[0];                                                  \
     else {                                                                                 \
-      _Pragma("unroll") for (int e = 0; e < EB; e += 2) {                                  \
-        const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                             \
-        rb[e] = y.x;                                                                       \
-        rb[e + 1] = y.y;                                                                   \
-      }                                                                                    \
+      _Pragma("unroll") for (int e = 0; e < EB; e += 2) ld2(pb + e, rb[e], rb[e + 1]);     \
     }                                                                                      \
     lastl = kl + 16 >= K;                                                                  \
     if (tl < cb.t1) {                                                                      \
@@ -409,7 +419,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
 // FP32 MFMA per task, FP64 across tasks.  v_mfma_f32_16x16x4f32 has the same
 // operand/result layout as the f64 variant above.
 // ---------------------------------------------------------------------------
-template <int CT, int RT>
+template <int CT, int RT, class TE = double>
 __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
 {
   const TileRec tile = p.tiles[blockIdx.x];
@@ -418,8 +428,8 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
   const int M = cb.m, N = cb.n;
   const int lane = threadIdx.x;
   const int li = lane & 15, lk = lane >> 4;
-  const double* A = static_cast<const double*>(p.A);
-  const double* B = static_cast<const double*>(p.B);
+  const TE* A = static_cast<const TE*>(p.A);
+  const TE* B = static_cast<const TE*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
   v4d acc[RT][CT];
 #pragma unroll
@@ -439,8 +449,8 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
     for (int t = cb.t0; t < cb.t1; ++t) {
       const TaskRec tr = trn;
       trn = p.tasks[min(t + 1, cb.t1 - 1)];
-      const double* Ab = A + tr.a_off;
-      const double* Bb = B + tr.b_off;
+      const TE* Ab = A + tr.a_off;
+      const TE* Bb = B + tr.b_off;
       const int K = tr.k;
       v4f part[RT][CT];
 #pragma unroll
@@ -540,7 +550,7 @@ __device__ inline void split3_cs(double xd, short& h, short& m, short& l)
 // tiles.  Loads are unconditional (clamped in-bounds addresses); k >= K is
 // zeroed before the split.
 constexpr int kW3Pitch = 40;
-template <int TR, int TC>
+template <int TR, int TC, class TE = double>
 __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
 {
   // waves: WR along the rows x WC along the columns of the tile
@@ -555,8 +565,8 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int wr = (TR / WR) * (w % WR), wc = (TC / WC) * (w / WR);
-  const double* A = static_cast<const double*>(p.A);
-  const double* B = static_cast<const double*>(p.B);
+  const TE* A = static_cast<const TE*>(p.A);
+  const TE* B = static_cast<const TE*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
   __shared__ short sA[2][3][TR][kW3Pitch];  // [stage][plane][row of the tile][k]
   __shared__ short sB[2][3][TC][kW3Pitch];  // [stage][plane][column of the tile][k]
@@ -589,15 +599,11 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
     k0a = kl + KA * ga;                                                                    \
     k0b = kl + KB * gb;                                                                    \
     Kl = K;                                                                                \
-    const double* pa = A + tr.a_off + ia;                                                  \
+    const TE* pa = A + tr.a_off + ia;                                                      \
     _Pragma("unroll") for (int e = 0; e < KA; ++e)                                         \
         ra[e] = pa[static_cast<int64_t>(min(k0a + e, K - 1)) * M];                         \
-    const double* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(k0b, K - 1);      \
-    _Pragma("unroll") for (int e = 0; e < KB; e += 2) {                                    \
-      const v2d8 y = *reinterpret_cast<const v2d8*>(pb + e);                               \
-      rb[e] = y.x;                                                                         \
-      rb[e + 1] = y.y;                                                                     \
-    }                                                                                      \
+    const TE* pb = B + tr.b_off + static_cast<int64_t>(jb) * K + min(k0b, K - 1);          \
+    _Pragma("unroll") for (int e = 0; e < KB; e += 2) ld2(pb + e, rb[e], rb[e + 1]);       \
     lastl = kl + 32 >= K;                                                                  \
     if (tl < cb.t1) {                                                                      \
       kl += 32;                                                                            \
@@ -809,6 +815,16 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
   if (a.ntiles <= 0) return;
   // one tile per workgroup of kclass_waves(k) waves
   const dim3 g(static_cast<unsigned>(a.ntiles)), b(kclass_waves(k) * kWave);
+  // the FP32 and split-BF16 classes read FP64 or FP32 operands
+#define SHT_EXEC_F32(kern, ...)                                                            \
+  {                                                                                        \
+    if (a.f32) hipLaunchKernelGGL((kern<__VA_ARGS__, float>), g, b, 0, s, a);              \
+    else hipLaunchKernelGGL((kern<__VA_ARGS__, double>), g, b, 0, s, a);                   \
+  }
+  SHT_REQUIRE(!a.f32 || k == KClass::F32x16h || k == KClass::F32W || k == KClass::F32N || k == KClass::F32M ||
+                  k == KClass::F32Nq || k == KClass::F32Mq || k == KClass::F32S || k == KClass::Bf16W ||
+                  k == KClass::Bf16S || k == KClass::Bf16N,
+              SHT_ERR_STATE, "FP32 operands reach an FP64 kernel class");
   switch (k) {
     case KClass::Exact8: hipLaunchKernelGGL(k_exact<8>, g, b, 0, s, a); break;
     case KClass::Exact16: hipLaunchKernelGGL(k_exact<16>, g, b, 0, s, a); break;
@@ -816,24 +832,25 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Mfma16: hipLaunchKernelGGL((k_mfma64<1, 4>), g, b, 0, s, a); break;
     case KClass::Mfma32: hipLaunchKernelGGL((k_mfma64<2, 4>), g, b, 0, s, a); break;
     case KClass::Mfma16h: hipLaunchKernelGGL((k_mfma64<1, 2>), g, b, 0, s, a); break;
-    case KClass::F32x16h: hipLaunchKernelGGL((k_mfma32<1, 2>), g, b, 0, s, a); break;
+    case KClass::F32x16h: SHT_EXEC_F32(k_mfma32, 1, 2); break;
     case KClass::Mfma16q: hipLaunchKernelGGL((k_mfma64<1, 8>), g, b, 0, s, a); break;
     case KClass::Mfma32q: hipLaunchKernelGGL((k_mfma64<2, 8>), g, b, 0, s, a); break;
     case KClass::Mfma64W: hipLaunchKernelGGL((k_mfmaw<0, 64, 64>), g, b, 0, s, a); break;
-    case KClass::F32W: hipLaunchKernelGGL((k_mfmaw<1, 64, 64>), g, b, 0, s, a); break;
+    case KClass::F32W: SHT_EXEC_F32(k_mfmaw, 1, 64, 64); break;
     case KClass::Mfma64S: hipLaunchKernelGGL((k_mfmaw<0, 32, 32>), g, b, 0, s, a); break;
     case KClass::Mfma64N: hipLaunchKernelGGL((k_mfmaw<0, 64, 16>), g, b, 0, s, a); break;
-    case KClass::F32N: hipLaunchKernelGGL((k_mfmaw<1, 64, 16>), g, b, 0, s, a); break;
+    case KClass::F32N: SHT_EXEC_F32(k_mfmaw, 1, 64, 16); break;
     case KClass::Mfma64M: hipLaunchKernelGGL((k_mfmaw<0, 64, 32>), g, b, 0, s, a); break;
-    case KClass::F32M: hipLaunchKernelGGL((k_mfmaw<1, 64, 32>), g, b, 0, s, a); break;
-    case KClass::F32Nq: hipLaunchKernelGGL((k_mfmaw<1, 128, 16>), g, b, 0, s, a); break;
-    case KClass::F32Mq: hipLaunchKernelGGL((k_mfmaw<1, 128, 32>), g, b, 0, s, a); break;
-    case KClass::F32S: hipLaunchKernelGGL((k_mfmaw<1, 32, 32>), g, b, 0, s, a); break;
-    case KClass::Bf16W: hipLaunchKernelGGL((k_bf3w<64, 64>), g, b, 0, s, a); break;
-    case KClass::Bf16S: hipLaunchKernelGGL((k_bf3w<32, 32>), g, b, 0, s, a); break;
-    case KClass::Bf16N: hipLaunchKernelGGL((k_bf3w<64, 16>), g, b, 0, s, a); break;
+    case KClass::F32M: SHT_EXEC_F32(k_mfmaw, 1, 64, 32); break;
+    case KClass::F32Nq: SHT_EXEC_F32(k_mfmaw, 1, 128, 16); break;
+    case KClass::F32Mq: SHT_EXEC_F32(k_mfmaw, 1, 128, 32); break;
+    case KClass::F32S: SHT_EXEC_F32(k_mfmaw, 1, 32, 32); break;
+    case KClass::Bf16W: SHT_EXEC_F32(k_bf3w, 64, 64); break;
+    case KClass::Bf16S: SHT_EXEC_F32(k_bf3w, 32, 32); break;
+    case KClass::Bf16N: SHT_EXEC_F32(k_bf3w, 64, 16); break;
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
+#undef SHT_EXEC_F32
   SHT_HIP(hipGetLastError());
 }
 

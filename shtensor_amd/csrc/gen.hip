@@ -101,6 +101,17 @@ __global__ __launch_bounds__(256) void k_gather_blocks(const uint32_t* in, uint3
   }
 }
 
+// Element-wise copy with conversion between tensors of one block pattern
+// (FP64 -> FP32 rounds to nearest even, FP32 -> FP64 is exact): the packed
+// layouts are identical, so the copy is one flat stream.
+template <class TI, class TO>
+__global__ __launch_bounds__(256) void k_convert(const TI* in, TO* out, int64_t n)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = static_cast<TO>(in[i]);
+}
+
 int grid_for(int64_t items, int64_t cap = 65536)
 {
   return static_cast<int>(items < 1 ? 1 : (items < cap ? items : cap));
@@ -136,6 +147,25 @@ void launch_gather_blocks(const void* in, void* out, const int64_t* d_recs, int6
   if (nrec <= 0) return;
   hipLaunchKernelGGL(k_gather_blocks, dim3(grid_for(nrec)), dim3(256), 0, s, static_cast<const uint32_t*>(in),
                      static_cast<uint32_t*>(out), d_recs, nrec);
+  SHT_HIP(hipGetLastError());
+}
+
+void launch_convert(const void* in, bool in_f64, void* out, bool out_f64, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return;
+  const int g = grid_for((n + 255) / 256, 16384);
+  if (in_f64 && !out_f64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, s, static_cast<const double*>(in),
+                       static_cast<float*>(out), n);
+  else if (!in_f64 && out_f64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(256), 0, s, static_cast<const float*>(in),
+                       static_cast<double*>(out), n);
+  else if (in_f64)
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(256), 0, s, static_cast<const double*>(in),
+                       static_cast<double*>(out), n);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), dim3(g), dim3(256), 0, s, static_cast<const float*>(in),
+                       static_cast<float*>(out), n);
   SHT_HIP(hipGetLastError());
 }
 

@@ -78,6 +78,8 @@ void launch_gen_fill(const TensorDesc& d, void* data, bool f64, uint64_t key, co
                      int nfactors, hipStream_t s);
 void launch_block_norms(const TensorDesc& d, const void* data, bool f64, int norm, double* out, hipStream_t s);
 void launch_zero(void* p, size_t bytes, hipStream_t s);
+// element-wise copy of n elements between FP64/FP32 buffers
+void launch_convert(const void* in, bool in_f64, void* out, bool out_f64, int64_t n, hipStream_t s);
 // d_recs: nrec triples (source word offset, destination word offset, words)
 void launch_gather_blocks(const void* in, void* out, const int64_t* d_recs, int64_t nrec, hipStream_t s);
 

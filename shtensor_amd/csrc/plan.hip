@@ -259,13 +259,16 @@ namespace sht {
 
 namespace {
 
-void check_tensors(const sht_tensor* A, const sht_tensor* B, const sht_tensor* C, sht_ctx* ctx)
+void check_tensors(const sht_tensor* A, const sht_tensor* B, const sht_tensor* C, sht_ctx* ctx, sht_precision prec)
 {
   SHT_REQUIRE(A && B && C, SHT_ERR_INVALID, "null tensor");
   SHT_REQUIRE(A->reserved && B->reserved, SHT_ERR_STATE, "A and B must be reserved");
   SHT_REQUIRE(A->ctx == ctx && B->ctx == ctx && C->ctx == ctx, SHT_ERR_INVALID, "tensors of another context");
-  SHT_REQUIRE(A->dtype == SHT_FLOAT64 && B->dtype == SHT_FLOAT64 && C->dtype == SHT_FLOAT64, SHT_ERR_UNSUPPORTED,
-              "contraction supports FLOAT64 tensors (precision forms select the block arithmetic)");
+  SHT_REQUIRE(C->dtype == SHT_FLOAT64, SHT_ERR_UNSUPPORTED,
+              "C must be FLOAT64 (the sum over contracted blocks runs in FP64 in every form)");
+  SHT_REQUIRE(A->dtype == B->dtype, SHT_ERR_UNSUPPORTED, "A and B must have the same float type");
+  SHT_REQUIRE(A->dtype == SHT_FLOAT64 || prec == SHT_PREC_FP32 || prec == SHT_PREC_BF16X3, SHT_ERR_UNSUPPORTED,
+              "FLOAT32 operands need the FP32 or BF16X3 form (the FP64 forms read FLOAT64 tensors)");
   SHT_REQUIRE(A != C && B != C, SHT_ERR_INVALID, "C must not alias A or B");
   // Tensor::filter without compress leaves negated ids (-id-1) in the index
   // list (Tensor.h:336-363); such a tensor has no valid block coordinates
@@ -522,9 +525,11 @@ void build_schedule(sht_plan* P)
   double sk = 0.0;
   for (int64_t c = 0; c < ncb; ++c) sk += static_cast<double>(w[c]);
   P->sum_k = sk;
-  const double es = 8.0;
-  P->bytes_in = es * (static_cast<double>(P->A->nelem) + static_cast<double>(P->B->nelem) +
-                      static_cast<double>(P->C->nelem));
+  // operands at their stored width (FP32 tensors for the FP32 and
+  // split-BF16 forms), C in FP64
+  const double ea = static_cast<double>(P->A->elem_size()), es = 8.0;
+  P->bytes_in = ea * (static_cast<double>(P->A->nelem) + static_cast<double>(P->B->nelem)) +
+                es * static_cast<double>(P->C->nelem);
   P->bytes_c = es * static_cast<double>(P->C->nelem);
   P->h_cblk = std::move(cb);
   P->h_work = std::move(w);
@@ -1218,8 +1223,8 @@ void setup_staging(sht_plan* P)
   order(m.b_row, m.b_col, P->orderB);
   order(m.c_row, m.c_col, P->orderC);
   Arena& arena = P->ctx->arena;
-  if (P->permA) P->d_Ap = arena.allocate(std::max<int64_t>(P->A->nelem, 1) * 8 + kReadSlack);
-  if (P->permB) P->d_Bp = arena.allocate(std::max<int64_t>(P->B->nelem, 1) * 8 + kReadSlack);
+  if (P->permA) P->d_Ap = arena.allocate(std::max<int64_t>(P->A->nelem, 1) * P->A->elem_size() + kReadSlack);
+  if (P->permB) P->d_Bp = arena.allocate(std::max<int64_t>(P->B->nelem, 1) * P->B->elem_size() + kReadSlack);
   if (P->permC) P->d_Cp = arena.allocate(std::max<int64_t>(P->C->nelem, 1) * 8 + kReadSlack);
 }
 
@@ -1242,7 +1247,7 @@ void permute(const sht_tensor* t, const void* in, void* out, const int* order, i
   pa.out = out;
   for (int i = 0; i < t->N; ++i) pa.order[i] = order[i];
   pa.inverse = inverse;
-  launch_permute(pa, true, s);
+  launch_permute(pa, t->dtype == SHT_FLOAT64, s);
 }
 
 }  // namespace
@@ -1295,7 +1300,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
     const bool zinit = bz && !az && P->as_all_touched;
     if (!zinit) launch_scale(static_cast<double*>(Cd), P->C->nelem, beta, bz ? 1 : 0, s);
     if (!az) {
-      launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, P->bt_total, s);
+      launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, Bd, P->B->dtype == SHT_FLOAT32, P->d_bt, P->bt_total, s);
       AsArgs g{};
       g.A = Ad;
       g.B = P->d_bt;
@@ -1311,6 +1316,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
       g.alpha_one = alpha == 1.0 ? 1 : 0;
       g.zero = zinit ? P->d_as_zero : nullptr;
       g.qctr = P->d_as_qctr;
+      g.f32 = P->A->dtype == SHT_FLOAT32 ? 1 : 0;
       if (ev) SHT_HIP(hipEventRecord((*ev)[0], s));
       launch_as(cls, g, s);
       if (ev) SHT_HIP(hipEventRecord((*ev)[1], s));
@@ -1329,6 +1335,7 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   a.alpha_zero = az ? 1 : 0;
   a.beta_zero = bz ? 1 : 0;
   a.alpha_one = alpha == 1.0 ? 1 : 0;
+  a.f32 = P->A->dtype == SHT_FLOAT32 ? 1 : 0;
   // Tile classes are independent (every C element belongs to one tile of one
   // class), so with several classes they run concurrently: the caller's
   // stream forks to up to three plan-owned streams and joins them again, so
@@ -1381,7 +1388,7 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
 {
   return guard([&] {
     SHT_REQUIRE(ctx && expr && out, SHT_ERR_INVALID, "null argument");
-    sht::check_tensors(A, B, C, ctx);
+    sht::check_tensors(A, B, C, ctx, prec);
     SHT_REQUIRE(prec >= SHT_PREC_FP64 && prec <= SHT_PREC_BF16X3, SHT_ERR_INVALID, "unknown precision");
     auto P = std::make_unique<sht_plan>();
     P->ctx = ctx;

@@ -65,6 +65,9 @@ struct ExecArgs {
   const TaskRec* tasks;
   double alpha, beta;
   int alpha_zero, beta_zero, alpha_one;
+  // A and B hold FP32 (the FP32 and split-BF16 forms read FP32 tensors);
+  // C is always FP64
+  int f32;
 };
 
 // kernel classes of the C-stationary launches
@@ -132,7 +135,7 @@ struct BtRec {
   int32_t pad;
 };
 enum class BtFormat : int { F64T = 0, F32T = 1, BF16X3 = 2 };
-void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const double* B, void* BT, int64_t bt_total,
+void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const void* B, bool b_f32, void* BT, int64_t bt_total,
                hipStream_t s);
 struct AsUnit {
   int32_t sweep0;  // first sweep of the block row
@@ -162,6 +165,9 @@ struct AsArgs {
   // workgroup tasks from the eight XCD queues of the unit list through these
   // eight counters (cleared by launch_as before every launch)
   unsigned long long* qctr;
+  // A (and the B the panels are made from) hold FP32: the FP32 and
+  // split-BF16 forms read FP32 tensors; C is always FP64
+  int f32;
 };
 constexpr int kAsZeroLen = 256;
 // Kernel classes.  The MFMA classes run 32-row units (two 16-row MFMA tiles

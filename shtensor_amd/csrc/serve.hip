@@ -227,7 +227,12 @@ void launch_group(sht_server* S, const std::vector<Request*>& g, Batch& b, hipSt
 {
   Arena& arena = S->ctx->arena;
   const char* base = static_cast<const char*>(S->ctx->d_base);
-  auto rel = [base](const void* p) { return static_cast<int64_t>((static_cast<const char*>(p) - base) / 8); };
+  // element offsets from the arena base (arena blocks are 256-byte aligned,
+  // so FP32 operands have whole FP32 offsets)
+  auto rel = [base](const void* p, int64_t w = 8) {
+    return static_cast<int64_t>((static_cast<const char*>(p) - base) / w);
+  };
+  const int64_t ew = g.front()->A->dtype == SHT_FLOAT32 ? 4 : 8;  // operand element bytes (the group shares them)
   int64_t nt = 0, ncb = 0;
   std::vector<int64_t> ntiles(static_cast<int>(KClass::Count), 0);
   for (Request* r : g) {
@@ -257,14 +262,14 @@ void launch_group(sht_server* S, const std::vector<Request*>& g, Batch& b, hipSt
     const void* Cd = P->permC ? P->d_Cp : P->C->d_data;
     if (P->ntasks)
       hipLaunchKernelGGL(k_rebase_tasks, dim3(nblk(P->ntasks)), dim3(256), 0, s, tasks + t0, P->d_tasks, P->ntasks,
-                         rel(Ad), rel(Bd));
+                         rel(Ad, ew), rel(Bd, ew));
     if (P->ncb)
       hipLaunchKernelGGL(k_rebase_cblk, dim3(nblk(P->ncb)), dim3(256), 0, s, cblk + c0, P->d_cblk, P->ncb, rel(Cd),
                          static_cast<int32_t>(t0));
     for (const auto& L : P->launches) {
       const int k = static_cast<int>(L.k);
       hipLaunchKernelGGL(k_rebase_tiles, dim3(nblk(L.count)), dim3(256), 0, s, tiles[k] + tfill[k],
-                         P->d_tiles + L.first, L.count, static_cast<int32_t>(c0), rel(Ad), rel(Bd), rel(Cd),
+                         P->d_tiles + L.first, L.count, static_cast<int32_t>(c0), rel(Ad, ew), rel(Bd, ew), rel(Cd),
                          static_cast<int32_t>(t0));
       tfill[k] += L.count;
     }
@@ -286,6 +291,7 @@ void launch_group(sht_server* S, const std::vector<Request*>& g, Batch& b, hipSt
   a.alpha_zero = ua == 0;
   a.beta_zero = ub == 0;
   a.alpha_one = first->alpha == 1.0;
+  a.f32 = ew == 4 ? 1 : 0;
   for (size_t k = 0; k < ntiles.size(); ++k) {
     if (!ntiles[k]) continue;
     a.tiles = tiles[k];
@@ -316,6 +322,7 @@ void launch_group_as(sht_server* S, const std::vector<Request*>& g, Batch& b, hi
   const Request* first = g.front();
   const bool f32 = first->prec == SHT_PREC_FP32;
   const int64_t tb = f32 ? 4 : 8;  // panel element bytes
+  const bool op32 = first->A->dtype == SHT_FLOAT32;  // FP32 operands (the group shares the type)
   auto rel = [base](const void* p, int64_t w) {
     return static_cast<int64_t>((static_cast<const char*>(p) - base) / w);
   };
@@ -353,10 +360,10 @@ void launch_group_as(sht_server* S, const std::vector<Request*>& g, Batch& b, hi
     if (zinit) zero = P->d_as_zero;
     if (!zinit) launch_scale(static_cast<double*>(Cd), P->C->nelem, first->beta, bz ? 1 : 0, s);
     if (!az) {
-      launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, static_cast<const double*>(Bd), P->d_bt, P->bt_total, s);
+      launch_bt(bt_format(P), P->d_bt_rec, P->bt_nrec, Bd, op32, P->d_bt, P->bt_total, s);
       if (P->as_nsweep)
         hipLaunchKernelGGL(k_rebase_sweeps, dim3(nblk(P->as_nsweep)), dim3(256), 0, s, sweeps + s0, P->d_as_sweep,
-                           P->as_nsweep, rel(Ad, 8), rel(P->d_bt, tb), c0);
+                           P->as_nsweep, rel(Ad, op32 ? 4 : 8), rel(P->d_bt, tb), c0);
       if (P->as_nccol)
         hipLaunchKernelGGL(k_rebase_ccol, dim3(nblk(P->as_nccol)), dim3(256), 0, s, ccol + c0, P->d_as_ccol,
                            P->as_nccol, rel(Cd, 8), zinit ? 1 : 0);
@@ -384,6 +391,7 @@ void launch_group_as(sht_server* S, const std::vector<Request*>& g, Batch& b, hi
     a.alpha_one = first->alpha == 1.0 ? 1 : 0;
     a.zero = zero;
     a.qctr = qctr;
+    a.f32 = op32 ? 1 : 0;
     launch_as(cls, a, s);
     S->nlaunches++;
   }
@@ -551,12 +559,12 @@ void dispatcher_loop(sht_server* S)
       }
     }
     if (!done.reqs.empty()) finish(S, done);
-    // C-stationary plans merge per (precision, alpha, beta), and so do
-    // plans that run the grouped A-stationary FP64/FP32 kernel; other
-    // A-stationary plans run alone
+    // C-stationary plans merge per (precision, operand type, alpha, beta),
+    // and so do plans that run the grouped A-stationary FP64/FP32 kernel;
+    // other A-stationary plans run alone
     std::vector<std::vector<Request*>> groups;
     {
-      std::map<std::tuple<int, int, uint64_t, uint64_t>, std::vector<Request*>> merged;
+      std::map<std::tuple<int, int, int, uint64_t, uint64_t>, std::vector<Request*>> merged;
       for (Request* r : runnable) {
         const bool as = r->plan->schedule >= 1;
         if (as && !as_mergeable(r)) {
@@ -566,7 +574,8 @@ void dispatcher_loop(sht_server* S)
         uint64_t ua, ub;
         std::memcpy(&ua, &r->alpha, 8);
         std::memcpy(&ub, &r->beta, 8);
-        merged[std::make_tuple(as ? 1 : 0, static_cast<int>(r->prec), ua, ub)].push_back(r);
+        merged[std::make_tuple(as ? 1 : 0, static_cast<int>(r->prec), static_cast<int>(r->A->dtype), ua, ub)]
+            .push_back(r);
       }
       for (auto& kv : merged) groups.push_back(std::move(kv.second));
     }

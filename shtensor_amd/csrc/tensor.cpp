@@ -424,6 +424,28 @@ sht_status sht_tensor_download(const sht_tensor* t, void* host, int64_t nelem)
   });
 }
 
+// No counterpart in the reference (its Tensor<T,N> fixes T per tensor):
+// dst takes src's block pattern and src's values converted to dst's float
+// type, so FP64 data can be given the FP32 copy the FP32 and split-BF16
+// contraction forms read.
+sht_status sht_tensor_convert(const sht_tensor* src, sht_tensor* dst, void* stream)
+{
+  return guard([&] {
+    SHT_REQUIRE(src && dst && src != dst, SHT_ERR_INVALID, "null or identical tensors");
+    SHT_REQUIRE(src->reserved, SHT_ERR_STATE, "source tensor not reserved");
+    SHT_REQUIRE(src->ctx == dst->ctx, SHT_ERR_INVALID, "tensors of another context");
+    SHT_REQUIRE(src->bs == dst->bs, SHT_ERR_INVALID, "block sizes differ");
+    SHT_REQUIRE(std::all_of(src->indices.begin(), src->indices.end(), [](int64_t id) { return id >= 0; }),
+                SHT_ERR_STATE, "tensor holds filtered blocks: compress the tensor first");
+    if (!dst->reserved || dst->indices != src->indices) {
+      SHT_REQUIRE(!dst->reserved, SHT_ERR_STATE, "destination reserved with another block pattern");
+      sht::tensor_reserve_sorted(dst, std::vector<int64_t>(src->indices));
+    }
+    sht::launch_convert(src->d_data, src->dtype == SHT_FLOAT64, dst->d_data, dst->dtype == SHT_FLOAT64, src->nelem,
+                        sht::as_stream(stream));
+  });
+}
+
 sht_status sht_tensor_block_norms(const sht_tensor* t, sht_block_norm norm, double* d_out, void* stream)
 {
   return guard([&] {
