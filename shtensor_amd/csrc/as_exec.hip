@@ -670,6 +670,33 @@ __device__ inline unsigned short bf16_bits(float x)
   return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
+// The split of eight consecutive contracted indices of one row (an MFMA
+// operand) with gfx950's packed conversion (v_cvt_pk_bf16_f32, round to
+// nearest even like bf16_bits): word e/2 of each plane holds the parts of
+// x[e] (low half) and x[e+1] (high half).
+typedef float v2f_s __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf_s __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u_s __attribute__((ext_vector_type(4)));
+__device__ inline void split3x8(const float (&x)[8], v8s& h, v8s& m, v8s& l)
+{
+  v4u_s hw, mw, lw;
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const v2f_s v = {x[e], x[e + 1]};
+    const v2bf_s hb = __builtin_convertvector(v, v2bf_s);
+    const v2f_s r1 = v - __builtin_convertvector(hb, v2f_s);
+    const v2bf_s mb = __builtin_convertvector(r1, v2bf_s);
+    const v2f_s r2 = r1 - __builtin_convertvector(mb, v2f_s);
+    const v2bf_s lb = __builtin_convertvector(r2, v2bf_s);
+    hw[e / 2] = __builtin_bit_cast(unsigned int, hb);
+    mw[e / 2] = __builtin_bit_cast(unsigned int, mb);
+    lw[e / 2] = __builtin_bit_cast(unsigned int, lb);
+  }
+  h = __builtin_bit_cast(v8s, hw);
+  m = __builtin_bit_cast(v8s, mw);
+  l = __builtin_bit_cast(v8s, lw);
+}
+
 __device__ inline void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l)
 {
   h = bf16_bits(x);
@@ -714,15 +741,12 @@ __device__ inline void sweep_bf3(const AsArgs& p, const SweepU& sw, int r0, int 
   for (int st = 0; st < nsteps; ++st) {
     v8s ah[R], am[R], al[R];
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int t = 0; t < R; ++t) {
+      float xf[8];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        unsigned short h, m, l;
-        split3(static_cast<float>(x_nx[t][e]), h, m, l);
-        ah[t][e] = static_cast<short>(h);
-        am[t][e] = static_cast<short>(m);
-        al[t][e] = static_cast<short>(l);
-      }
+      for (int e = 0; e < 8; ++e) xf[e] = static_cast<float>(x_nx[t][e]);
+      split3x8(xf, ah[t], am[t], al[t]);
+    }
     load_a8(min(st + 1, nsteps - 1));
     const int k0 = kBfK * st;
 #pragma unroll
@@ -909,16 +933,12 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
   auto step = [&](int st, const TE (&x)[R][8], int stg) {
     v8s ah[R], am[R], al[R];
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int t = 0; t < R; ++t) {
+      float xf[8];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int kk = kBfK * st + 8 * lk + e;
-        unsigned short h, m, l;
-        split3(kk < K ? static_cast<float>(x[t][e]) : 0.f, h, m, l);
-        ah[t][e] = static_cast<short>(h);
-        am[t][e] = static_cast<short>(m);
-        al[t][e] = static_cast<short>(l);
-      }
+      for (int e = 0; e < 8; ++e) xf[e] = kBfK * st + 8 * lk + e < K ? static_cast<float>(x[t][e]) : 0.f;
+      split3x8(xf, ah[t], am[t], al[t]);
+    }
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const v8s bh = __builtin_bit_cast(v8s, stage[stg][0][q][lane]);

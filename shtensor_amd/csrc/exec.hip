@@ -517,26 +517,27 @@ __global__ __launch_bounds__(64) void k_mfma32(ExecArgs p)
 // Three-term split-BF16 (C-stationary).  v_mfma_f32_16x16x32_bf16 (wave64):
 // A frag lane l holds A[l%16][8*(l/16) .. 8*(l/16)+7]; B frag holds
 // B[8*(l/16)..+7][l%16]; D as the f32 form above.  x = hi + mid + lo (bf16
-// each, split3_cs below); the six products of order <= 2 accumulate in
+// each, split3x2_cs below); the six products of order <= 2 accumulate in
 // FP32 per task (smallest first: lo*hi, mid*mid, hi*lo, mid*hi, hi*mid,
 // hi*hi), FP64 across tasks.
 // ---------------------------------------------------------------------------
-__device__ inline unsigned short bf16_bits_cs(float x)
+// x = hi + mid + lo for two values at once with gfx950's packed conversion
+// (v_cvt_pk_bf16_f32, round to nearest even): hi = bf16(x), mid =
+// bf16(x - hi), lo = bf16(x - hi - mid), each difference exact in FP32; each
+// output word holds the parts of x0 (low half) and x1 (high half).
+typedef float v2f_cs __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf_cs __attribute__((ext_vector_type(2)));
+__device__ inline void split3x2_cs(float x0, float x1, uint32_t& h, uint32_t& m, uint32_t& l)
 {
-  const unsigned int u = __float_as_uint(x);
-  return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
-__device__ inline void split3_cs(double xd, short& h, short& m, short& l)
-{
-  const float x = static_cast<float>(xd);
-  const unsigned short hb = bf16_bits_cs(x);
-  const float r1 = x - __uint_as_float(static_cast<unsigned int>(hb) << 16);
-  const unsigned short mb = bf16_bits_cs(r1);
-  const float r2 = r1 - __uint_as_float(static_cast<unsigned int>(mb) << 16);
-  h = static_cast<short>(hb);
-  m = static_cast<short>(mb);
-  l = static_cast<short>(bf16_bits_cs(r2));
+  const v2f_cs x = {x0, x1};
+  const v2bf_cs hb = __builtin_convertvector(x, v2bf_cs);
+  const v2f_cs r1 = x - __builtin_convertvector(hb, v2f_cs);
+  const v2bf_cs mb = __builtin_convertvector(r1, v2bf_cs);
+  const v2f_cs r2 = r1 - __builtin_convertvector(mb, v2f_cs);
+  const v2bf_cs lb = __builtin_convertvector(r2, v2bf_cs);
+  h = __builtin_bit_cast(uint32_t, hb);
+  m = __builtin_bit_cast(uint32_t, mb);
+  l = __builtin_bit_cast(uint32_t, lb);
 }
 
 // Split-BF16 on TR x TC C tiles, one workgroup of four waves per tile (64 x
@@ -616,26 +617,28 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
   }
 #define SHT_W3_STORE(stg)                                                                  \
   {                                                                                        \
-    typedef short vas __attribute__((ext_vector_type(KA)));                                \
-    typedef short vbs __attribute__((ext_vector_type(KB)));                                \
+    typedef uint32_t vas __attribute__((ext_vector_type(KA / 2)));                         \
+    typedef uint32_t vbs __attribute__((ext_vector_type(KB / 2)));                         \
     vas h, m, l;                                                                           \
-    _Pragma("unroll") for (int e = 0; e < KA; ++e) {                                       \
-      short x, y, z;                                                                       \
-      split3_cs(k0a + e < Kl ? ra[e] : 0.0, x, y, z);                                      \
-      h[e] = x;                                                                            \
-      m[e] = y;                                                                            \
-      l[e] = z;                                                                            \
+    _Pragma("unroll") for (int e = 0; e < KA; e += 2) {                                    \
+      uint32_t x, y, z;                                                                    \
+      split3x2_cs(k0a + e < Kl ? static_cast<float>(ra[e]) : 0.f,                          \
+                  k0a + e + 1 < Kl ? static_cast<float>(ra[e + 1]) : 0.f, x, y, z);         \
+      h[e / 2] = x;                                                                        \
+      m[e / 2] = y;                                                                        \
+      l[e / 2] = z;                                                                        \
     }                                                                                      \
     *reinterpret_cast<vas*>(&sA[stg][0][ra_][KA * ga]) = h;                                \
     *reinterpret_cast<vas*>(&sA[stg][1][ra_][KA * ga]) = m;                                \
     *reinterpret_cast<vas*>(&sA[stg][2][ra_][KA * ga]) = l;                                \
     vbs hb, mb, lb;                                                                        \
-    _Pragma("unroll") for (int e = 0; e < KB; ++e) {                                       \
-      short x, y, z;                                                                       \
-      split3_cs(k0b + e < Kl ? rb[e] : 0.0, x, y, z);                                      \
-      hb[e] = x;                                                                           \
-      mb[e] = y;                                                                           \
-      lb[e] = z;                                                                           \
+    _Pragma("unroll") for (int e = 0; e < KB; e += 2) {                                    \
+      uint32_t x, y, z;                                                                    \
+      split3x2_cs(k0b + e < Kl ? static_cast<float>(rb[e]) : 0.f,                          \
+                  k0b + e + 1 < Kl ? static_cast<float>(rb[e + 1]) : 0.f, x, y, z);         \
+      hb[e / 2] = x;                                                                       \
+      mb[e / 2] = y;                                                                       \
+      lb[e / 2] = z;                                                                       \
     }                                                                                      \
     *reinterpret_cast<vbs*>(&sB[stg][0][cb_][KB * gb]) = hb;                               \
     *reinterpret_cast<vbs*>(&sB[stg][1][cb_][KB * gb]) = mb;                               \
