@@ -35,22 +35,23 @@ def peak_of(form):
 
 HBM_PEAK = 8.0e12
 # kernel class reported by sht_plan_execute_timed -> substring of the kernel's
-# symbol in rocprofv3's output
+# symbol in rocprofv3's output (the leading template arguments: the FP32 and
+# split-BF16 kernels carry the operand type, double or float, last)
 KERNEL_SYMBOL = {
-    "as_mfma64": "k_asp<2, 8, 0, 1, true>", "as_mfma64_g": "k_asl<0, 4, true, 4>",
-    "as_mfma32": "k_asp<2, 8, 1, 1, false>", "as_mfma32_g": "k_asl<1, 4, false, 4>", "as_bf16x3": "k_asp<2, 8, 2, 1, false>",
-    "as_bf16x3_g": "k_asb", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true>",
-    "as_mfma32_n4": "k_asp<4, 4, 1, 1, false>",
-    "as_exact": "k_as_exact", "exact_n8": "k_exact<8>", "exact_n16": "k_exact<16>", "exact_n32": "k_exact<32>",
-    "mfma64_n16": "k_mfma64<1, 4>", "mfma64_n32": "k_mfma64<2, 4>",
-    "mfma64_n16_r32": "k_mfma64<1, 2>", "mfma32_n16_r32": "k_mfma32<1, 2>",
-    "mfma64_n64w": "k_mfmaw<0, 64, 64>", "mfma32_n64w": "k_mfmaw<1, 64, 64>", "bf16x3_n64w": "k_bf3w<64, 64>",
-    "mfma64_n32s": "k_mfmaw<0, 32, 32>", "mfma32_n32s": "k_mfmaw<1, 32, 32>",
-    "mfma64_n16w": "k_mfmaw<0, 64, 16>", "mfma32_n16w": "k_mfmaw<1, 64, 16>",
-    "mfma64_n32w": "k_mfmaw<0, 64, 32>", "mfma32_n32w": "k_mfmaw<1, 64, 32>",
-    "mfma32_n16w_r128": "k_mfmaw<1, 128, 16>", "mfma32_n32w_r128": "k_mfmaw<1, 128, 32>", "bf16x3_n32s": "k_bf3w<32, 32>",
-    "bf16x3_n16w": "k_bf3w<64, 16>",
-    "mfma64_n16_r128": "k_mfma64<1, 8>", "mfma64_n32_r128": "k_mfma64<2, 8>",
+    "as_mfma64": "k_asp<2, 8, 0, 1, true", "as_mfma64_g": "k_asl<0, 4, true, 4",
+    "as_mfma32": "k_asp<2, 8, 1, 1, false", "as_mfma32_g": "k_asl<1, 4, false, 4", "as_bf16x3": "k_asp<2, 8, 2, 1, false",
+    "as_bf16x3_g": "k_asb", "as_mfma64_n4": "k_asp<4, 4, 0, 1, true",
+    "as_mfma32_n4": "k_asp<4, 4, 1, 1, false",
+    "as_exact": "k_as_exact", "exact_n8": "k_exact<8", "exact_n16": "k_exact<16", "exact_n32": "k_exact<32",
+    "mfma64_n16": "k_mfma64<1, 4", "mfma64_n32": "k_mfma64<2, 4",
+    "mfma64_n16_r32": "k_mfma64<1, 2", "mfma32_n16_r32": "k_mfma32<1, 2",
+    "mfma64_n64w": "k_mfmaw<0, 64, 64", "mfma32_n64w": "k_mfmaw<1, 64, 64", "bf16x3_n64w": "k_bf3w<64, 64",
+    "mfma64_n32s": "k_mfmaw<0, 32, 32", "mfma32_n32s": "k_mfmaw<1, 32, 32",
+    "mfma64_n16w": "k_mfmaw<0, 64, 16", "mfma32_n16w": "k_mfmaw<1, 64, 16",
+    "mfma64_n32w": "k_mfmaw<0, 64, 32", "mfma32_n32w": "k_mfmaw<1, 64, 32",
+    "mfma32_n16w_r128": "k_mfmaw<1, 128, 16", "mfma32_n32w_r128": "k_mfmaw<1, 128, 32", "bf16x3_n32s": "k_bf3w<32, 32",
+    "bf16x3_n16w": "k_bf3w<64, 16",
+    "mfma64_n16_r128": "k_mfma64<1, 8", "mfma64_n32_r128": "k_mfma64<2, 8",
 }
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]
 

@@ -280,13 +280,25 @@ class Tensor
  * the reference declares but leaves empty, src/Tensor/Tensor.cpp).  An empty
  * C gets the product's block pattern; a reserved C keeps its own.  Done when
  * it returns. */
-template <int NA, int NB, int NC>
-void contract(Context& ctx, const std::string& expr, double alpha, const Tensor<double, NA>& A,
-              const Tensor<double, NB>& B, double beta, Tensor<double, NC>& C, Precision prec = Precision::FP64,
+template <class TO, int NA, int NB, int NC>
+void contract(Context& ctx, const std::string& expr, double alpha, const Tensor<TO, NA>& A,
+              const Tensor<TO, NB>& B, double beta, Tensor<double, NC>& C, Precision prec = Precision::FP64,
               void* stream = nullptr)
 {
+  // operands are double for every form, or float for Precision::FP32 and
+  // Precision::BF16X3 (the library rejects other combinations); C is double
   check(sht_contract(ctx.handle(), expr.c_str(), alpha, A.handle(), B.handle(), beta, C.handle(),
                      static_cast<sht_precision>(prec), stream));
+}
+
+/* No counterpart in the reference: dst takes src's block pattern and src's
+ * values converted to U (sht_tensor_convert; double -> float rounds to
+ * nearest even).  Done when it returns. */
+template <class U, class T, int N>
+void convert(const Tensor<T, N>& src, Tensor<U, N>& dst)
+{
+  check(sht_tensor_convert(src.handle(), dst.handle(), nullptr));
+  check(sht_device_synchronize());
 }
 
 }  // namespace Shtensor

@@ -470,15 +470,36 @@ KClass quad_rows(KClass k)
 
 void build_tiles(sht_plan* P);
 
-// Each tile record's copy of its C block record and of its first task's
-// operand offsets (padding tiles stay empty).
-__global__ void k_tile_fill(TileRec* t, int64_t n, const CBlkRec* cblk, const TaskRec* tasks)
+// Tile placement from the host's plan of the tile list: tile (C block, first
+// row, first column) goes to position pos; every other position holds a
+// padding tile.
+struct TilePlace {
+  int64_t pos;
+  int32_t cb, m0, n0, unused;
+};
+
+__global__ void k_tile_pads(TileRec* t, int64_t n)
 {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  TileRec r = t[i];
-  if (r.pad) return;
-  const CBlkRec c = cblk[r.cb];
+  TileRec r{};
+  r.pad = 1;
+  t[i] = r;
+}
+
+// Each placed tile's record: its C block record and its first task's operand
+// offsets copied in.
+__global__ void k_tile_place(TileRec* t, const TilePlace* items, int64_t nitems, const CBlkRec* cblk,
+                             const TaskRec* tasks)
+{
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nitems) return;
+  const TilePlace p = items[i];
+  TileRec r{};
+  r.cb = p.cb;
+  r.m0 = p.m0;
+  r.n0 = p.n0;
+  const CBlkRec c = cblk[p.cb];
   r.c_off = c.c_off;
   r.m = c.m;
   r.n = c.n;
@@ -490,7 +511,7 @@ __global__ void k_tile_fill(TileRec* t, int64_t n, const CBlkRec* cblk, const Ta
     r.b0 = k.b_off;
     r.k0 = k.k;
   }
-  t[i] = r;
+  t[p.pos] = r;
 }
 
 void build_schedule(sht_plan* P)
@@ -566,11 +587,6 @@ void build_tiles(sht_plan* P)
   // columns of B -- in launches of at least 32 waves per SIMD; shorter ones
   // (a few waves per SIMD, where the last wave to finish sets the time) order their
   // tiles by descending cost instead.
-  struct T {
-    TileRec r;
-    double cost;
-    int64_t row;  // group key: C's block row (A the larger operand), else its square
-  };
   const bool by_row = P->A->nelem >= 2 * P->B->nelem;
   auto axes_key = [&](int64_t id, const std::vector<int>& axes) {
     int64_t k = 0, mul = 1;
@@ -587,7 +603,8 @@ void build_tiles(sht_plan* P)
     if (by_row) return axes_key(id, m.c_row);
     return (axes_key(id, m.c_row) / kSq) * (ncolkeys / kSq + 1) + axes_key(id, m.c_col) / kSq;
   };
-  std::vector<std::vector<T>> per(static_cast<int>(KClass::Count));
+  constexpr int NK = static_cast<int>(KClass::Count);
+  constexpr int kXcd = 8;
   const bool exact = P->prec == SHT_PREC_FP64_EXACT;
   // 128-row FP64 / FP32 tiles only for large launches (more than 64 64-row tiles
   // per SIMD): there a wave's fixed cost (the dependent record loads before
@@ -597,100 +614,175 @@ void build_tiles(sht_plan* P)
   const bool quad_ok = P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32;
   const bool quad = quad_ok && (variant == 2 || (variant != 1 && variant != 4 && n64 > 64 * 1024));
   const bool uniform = variant == 4;
-  for (int64_t c = 0; c < ncb; ++c) {
-    const int M = cb[c].m, N = cb[c].n;
-    KClass k;
+  // the tile class of C block c (its row tiles may take the 32- or 128-row
+  // form of it)
+  auto main_class = [&](int64_t c) {
+    const int N = cb[c].n;
     if (uniform)
-      k = exact ? KClass::Exact32
-                : (P->prec == SHT_PREC_FP32 ? KClass::F32W
-                                            : (P->prec == SHT_PREC_BF16X3 ? KClass::Bf16W : KClass::Mfma64W));
-    else if (exact) k = N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
-    else if (P->prec == SHT_PREC_FP32) k = N <= 16 ? KClass::F32N : (N <= 32 ? KClass::F32M : KClass::F32W);
-    else if (P->prec == SHT_PREC_BF16X3) k = N <= 16 ? KClass::Bf16N : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
-    else if (N > 32) k = KClass::Mfma64W;
+      return exact ? KClass::Exact32
+                   : (P->prec == SHT_PREC_FP32 ? KClass::F32W
+                                               : (P->prec == SHT_PREC_BF16X3 ? KClass::Bf16W : KClass::Mfma64W));
+    if (exact) return N <= 8 ? KClass::Exact8 : (N <= 16 ? KClass::Exact16 : KClass::Exact32);
+    if (P->prec == SHT_PREC_FP32) return N <= 16 ? KClass::F32N : (N <= 32 ? KClass::F32M : KClass::F32W);
+    if (P->prec == SHT_PREC_BF16X3) return N <= 16 ? KClass::Bf16N : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
+    if (N > 32) return KClass::Mfma64W;
     // (FP64 large launches: one-wave 64-row tiles beside the 128-row ones,
     // measured faster there than the four-wave 64-row forms)
-    else if (quad) k = N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
-    else k = N <= 16 ? KClass::Mfma64N : KClass::Mfma64M;
+    if (quad) return N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
+    return N <= 16 ? KClass::Mfma64N : KClass::Mfma64M;
+  };
+  // Calls f(class, m0, n0, cost) for the tiles of C block c in order: row
+  // tiles of 128 rows while more than 96 remain (FP64 and FP32 up to 32
+  // columns, large launches), then 64; the last row tile of an MFMA class
+  // takes the 32-row form when it can.
+  auto each_tile = [&](int64_t c, auto&& f) {
+    const int M = cb[c].m, N = cb[c].n;
+    const KClass k = main_class(c);
     const int cols = kclass_cols(k);
-    const int64_t rk = row_key(P->C->indices[c]);
     for (int m0 = 0; m0 < M;) {
-      // row tiles: 128 rows while more than 96 remain (FP64 and FP32 up to
-      // 32 columns, large launches), then 64; the last row tile of an MFMA class takes the
-      // 32-row form when it can
       KClass kt = k;
       if (uniform) kt = k;
       else if (!exact && M - m0 <= 32) kt = half_rows(k);
       else if (quad && M - m0 > 96) kt = quad_rows(k);
-      for (int n0 = 0; n0 < N; n0 += cols) {
-        const double cost =
-            static_cast<double>(std::min(kclass_rows(kt), M - m0)) * std::min(cols, N - n0) * (w[c] + 1);
-        TileRec tr{};
-        tr.cb = static_cast<int32_t>(c);
-        tr.m0 = m0;
-        tr.n0 = n0;
-        per[static_cast<int>(kt)].push_back({tr, cost, rk});
-      }
-      m0 += kclass_rows(kt);
+      const int rows = kclass_rows(kt);
+      for (int n0 = 0; n0 < N; n0 += cols)
+        f(static_cast<int>(kt), m0, n0,
+          static_cast<double>(std::min(rows, M - m0)) * std::min(cols, N - n0) * (w[c] + 1));
+      m0 += rows;
     }
+  };
+  auto place = [](int64_t pos, int64_t c, int m0, int n0) {
+    return TilePlace{pos, static_cast<int32_t>(c), m0, n0, 0};
+  };
+  std::vector<int64_t> count(NK, 0);
+  for (int64_t c = 0; c < ncb; ++c) each_tile(c, [&](int k, int, int, double) { ++count[k]; });
+  // Classes ordered by descending tile cost (variant 3, and small launches
+  // when A and B are alike) and classes dealt to the XCD queues by group
+  std::vector<char> by_cost(NK, 0);
+  for (int k = 0; k < NK; ++k)
+    by_cost[k] = count[k] > 0 && (variant == 3 || (!by_row && count[k] * kclass_waves(static_cast<KClass>(k)) < 32 * 1024));
+  struct CostTile {
+    int64_t c;
+    int32_t m0, n0;
+    double cost;
+  };
+  std::vector<std::vector<CostTile>> ct(NK);
+  for (int k = 0; k < NK; ++k)
+    if (by_cost[k]) ct[k].reserve(static_cast<size_t>(count[k]));
+  if (std::any_of(by_cost.begin(), by_cost.end(), [](char b) { return b != 0; }))
+    for (int64_t c = 0; c < ncb; ++c)
+      each_tile(c, [&](int k, int m0, int n0, double cost) {
+        if (by_cost[k]) ct[k].push_back({c, m0, n0, cost});
+      });
+  // The grouped classes: C blocks in group-key order (a stable counting sort;
+  // keys are bounded by the number of C block rows or squares), so the tiles
+  // of one group come in C block order.  A group's tiles are dealt to one
+  // XCD queue, groups by descending total cost to the least-loaded queue,
+  // and the eight queues interleaved with padding tiles.
+  std::vector<int64_t> key(static_cast<size_t>(ncb)), corder(static_cast<size_t>(ncb));
+  int64_t kmax = 0;
+  for (int64_t c = 0; c < ncb; ++c) kmax = std::max(kmax, key[c] = row_key(P->C->indices[c]));
+  if (kmax < 8 * ncb + 4096) {
+    std::vector<int64_t> start(static_cast<size_t>(kmax) + 2, 0);
+    for (int64_t c = 0; c < ncb; ++c) ++start[static_cast<size_t>(key[c]) + 1];
+    for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
+    for (int64_t c = 0; c < ncb; ++c) corder[start[static_cast<size_t>(key[c])]++] = c;
+  } else {
+    std::iota(corder.begin(), corder.end(), int64_t{0});
+    std::stable_sort(corder.begin(), corder.end(), [&](int64_t x, int64_t y) { return key[x] < key[y]; });
   }
-  constexpr int kXcd = 8;
-  std::vector<TileRec> all;
+  struct Group {
+    int64_t key, n;  // tiles of the class in the group
+    double cost;
+    int x;           // XCD queue
+    int64_t slot0;   // the group's first slot in its queue
+  };
+  std::vector<std::vector<Group>> grp(NK);
+  for (int64_t c : corder)
+    each_tile(c, [&](int k, int, int, double cost) {
+      if (by_cost[k]) return;
+      auto& g = grp[k];
+      if (g.empty() || g.back().key != key[c]) g.push_back({key[c], 0, 0.0, 0, 0});
+      ++g.back().n;
+      g.back().cost += cost;
+    });
+  // where each class's launch starts in the tile list, and its length
+  std::vector<int64_t> first(NK, 0), len(NK, 0);
+  int64_t total = 0;
   P->launches.clear();
-  for (int k = 0; k < static_cast<int>(KClass::Count); ++k) {
-    auto& v = per[k];
-    if (v.empty()) continue;
-    if (variant == 3 || (!by_row && static_cast<int64_t>(v.size()) * kclass_waves(static_cast<KClass>(k)) < 32 * 1024)) {
-      std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.cost > y.cost; });
-      double ck = 0.0;
-      for (const T& t : v) ck += t.cost;
-      P->launches.push_back(
-          {static_cast<KClass>(k), static_cast<int64_t>(all.size()), static_cast<int64_t>(v.size()), ck});
-      for (const T& t : v) all.push_back(t.r);
-      continue;
-    }
-    // groups (rows or squares): tiles grouped by key, groups by descending
-    // total cost
-    std::stable_sort(v.begin(), v.end(), [](const T& x, const T& y) { return x.row < y.row; });
-    struct RowSpan {
-      size_t b, e;
-      double cost;
-    };
-    std::vector<RowSpan> rows;
-    for (size_t i = 0; i < v.size();) {
-      size_t e = i;
-      double c = 0.0;
-      while (e < v.size() && v[e].row == v[i].row) c += v[e++].cost;
-      rows.push_back({i, e, c});
-      i = e;
-    }
-    std::stable_sort(rows.begin(), rows.end(), [](const RowSpan& x, const RowSpan& y) { return x.cost > y.cost; });
-    std::vector<std::vector<TileRec>> q(kXcd);
-    std::vector<double> load(kXcd, 0.0);
-    for (const RowSpan& r : rows) {
-      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-      for (size_t i = r.b; i < r.e; ++i) q[x].push_back(v[i].r);
-      load[x] += r.cost;
-    }
-    size_t longest = 0;
-    for (const auto& qq : q) longest = std::max(longest, qq.size());
-    const int64_t first = static_cast<int64_t>(all.size());
-    for (size_t slot = 0; slot < longest; ++slot)
-      for (int x = 0; x < kXcd; ++x) {
-        TileRec padt{};
-        padt.pad = 1;
-        all.push_back(slot < q[x].size() ? q[x][slot] : padt);
-      }
-    while (!all.empty() && static_cast<int64_t>(all.size()) > first && all.back().pad) all.pop_back();
+  for (int k = 0; k < NK; ++k) {
+    if (!count[k]) continue;
     double ck = 0.0;
-    for (const RowSpan& r : rows) ck += r.cost;
-    P->launches.push_back({static_cast<KClass>(k), first, static_cast<int64_t>(all.size()) - first, ck});
+    if (by_cost[k]) {
+      len[k] = count[k];
+      for (const CostTile& t : ct[k]) ck += t.cost;
+    } else {
+      auto& g = grp[k];
+      std::vector<int64_t> go(g.size());
+      std::iota(go.begin(), go.end(), int64_t{0});
+      std::stable_sort(go.begin(), go.end(), [&](int64_t a, int64_t b) { return g[a].cost > g[b].cost; });
+      std::vector<double> load(kXcd, 0.0);
+      std::vector<int64_t> qlen(kXcd, 0);
+      for (int64_t gi : go) {
+        const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        g[gi].x = x;
+        g[gi].slot0 = qlen[x];
+        qlen[x] += g[gi].n;
+        load[x] += g[gi].cost;
+        ck += g[gi].cost;
+      }
+      // (the last slot's trailing padding tiles are left out)
+      const int64_t longest = *std::max_element(qlen.begin(), qlen.end());
+      int64_t last = 0;
+      for (int x = 0; x < kXcd; ++x)
+        if (qlen[x] == longest) last = x;
+      len[k] = (longest - 1) * kXcd + last + 1;
+    }
+    first[k] = total;
+    total += len[k];
+    P->launches.push_back({static_cast<KClass>(k), first[k], len[k], ck});
   }
-  if (!all.empty()) {
-    P->d_tiles = static_cast<TileRec*>(arena.allocate(all.size() * sizeof(TileRec)));
-    SHT_HIP(hipMemcpy(P->d_tiles, all.data(), all.size() * sizeof(TileRec), hipMemcpyHostToDevice));
-    const int64_t n = static_cast<int64_t>(all.size());
-    hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n)), dim3(256), 0, nullptr, P->d_tiles, n, P->d_cblk, P->d_tasks);
+  // the tiles' positions, written in walk order (the device scatters them)
+  int64_t nreal = 0;
+  for (int k = 0; k < NK; ++k) nreal += count[k];
+  std::vector<TilePlace> items;
+  items.reserve(static_cast<size_t>(nreal));
+  for (int k = 0; k < NK; ++k) {
+    if (!by_cost[k]) continue;
+    // descending cost, ties in walk order (the order of a stable sort)
+    std::vector<std::pair<double, int64_t>> o(ct[k].size());
+    for (size_t i = 0; i < o.size(); ++i) o[i] = {-ct[k][i].cost, static_cast<int64_t>(i)};
+    std::sort(o.begin(), o.end());
+    for (size_t i = 0; i < o.size(); ++i) {
+      const CostTile& t = ct[k][o[i].second];
+      items.push_back(place(first[k] + static_cast<int64_t>(i), t.c, t.m0, t.n0));
+    }
+  }
+  {
+    // second walk in the same order: each group's tiles go to consecutive
+    // slots of its queue (tile j of a group at slot slot0 + j)
+    std::vector<int64_t> gi(NK, -1), pos(NK, 0);
+    std::vector<int64_t> gkey(NK, -1);
+    for (int64_t c : corder)
+      each_tile(c, [&](int k, int m0, int n0, double) {
+        if (by_cost[k]) return;
+        if (gkey[k] != key[c] || gi[k] < 0) {
+          ++gi[k];
+          gkey[k] = key[c];
+          pos[k] = 0;
+        }
+        const Group& g = grp[k][gi[k]];
+        items.push_back(place(first[k] + (g.slot0 + pos[k]++) * kXcd + g.x, c, m0, n0));
+      });
+  }
+  if (total > 0) {
+    P->d_tiles = static_cast<TileRec*>(arena.allocate(total * sizeof(TileRec)));
+    Scratch S(arena);
+    TilePlace* d_items = S.get<TilePlace>(std::max<int64_t>(nreal, 1));
+    SHT_HIP(hipMemcpy(d_items, items.data(), items.size() * sizeof(TilePlace), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_tile_pads, dim3(blocks_for(total)), dim3(256), 0, nullptr, P->d_tiles, total);
+    hipLaunchKernelGGL(k_tile_place, dim3(blocks_for(nreal)), dim3(256), 0, nullptr, P->d_tiles, d_items, nreal,
+                       P->d_cblk, P->d_tasks);
     SHT_HIP(hipGetLastError());
     SHT_HIP(hipStreamSynchronize(nullptr));
   }

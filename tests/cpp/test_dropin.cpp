@@ -285,6 +285,39 @@ void test_contract_flow(Shtensor::Precision prec, const char* name)
   report(ok, std::string("reserve -> contract -> download (") + name + "), " + std::to_string(nt) + " tasks");
 }
 
+// FP32 operands (Tensor<float,N>, the reference's T = float) for the FP32
+// and split-BF16 forms: convert -> contract gives the same C bit for bit as
+// the same form reading the double tensors the floats were rounded from.
+void test_fp32_operands(Shtensor::Precision prec, const char* name)
+{
+  Shtensor::Context ctx(256 << 20);
+  const std::vector<int> bi = {16, 5, 21, 9}, bj = {7, 13, 28}, bk = {11, 16, 5, 9, 15}, bl = {28, 7, 21};
+  Shtensor::VArray<3> bsa = {bi, bj, bk};
+  Shtensor::VArray<2> bsb = {bk, bl};
+  Shtensor::VArray<3> bsc = {bi, bj, bl};
+  Shtensor::Tensor<double, 3> A(ctx, bsa);
+  Shtensor::Tensor<double, 2> B(ctx, bsb);
+  A.reserve_all();
+  B.reserve_all();
+  std::vector<double> ha(A.nelem()), hb(B.nelem());
+  fill_random(ha, 21);
+  fill_random(hb, 22);
+  A.upload(ha);
+  B.upload(hb);
+  Shtensor::Tensor<float, 3> A32(ctx, bsa);
+  Shtensor::Tensor<float, 2> B32(ctx, bsb);
+  Shtensor::convert(A, A32);
+  Shtensor::convert(B, B32);
+  const std::vector<float> a32 = A32.download();
+  bool ok = a32.size() == ha.size();
+  for (size_t i = 0; ok && i < ha.size(); ++i) ok = a32[i] == static_cast<float>(ha[i]);
+  Shtensor::Tensor<double, 3> C64(ctx, bsc), C32(ctx, bsc);
+  Shtensor::contract(ctx, "ijk,kl->ijl", 1.0, A, B, 0.0, C64, prec);
+  Shtensor::contract(ctx, "ijk,kl->ijl", 1.0, A32, B32, 0.0, C32, prec);
+  ok = ok && same_bits(C32.download(), C64.download());
+  report(ok, std::string("convert -> FP32 operands -> contract (") + name + ") == double operands");
+}
+
 }  // namespace
 
 int main()
@@ -319,6 +352,8 @@ int main()
     test_tensor();
     test_contract_flow(Shtensor::Precision::FP64, "FP64");
     test_contract_flow(Shtensor::Precision::FP64_EXACT, "FP64_EXACT");
+    test_fp32_operands(Shtensor::Precision::FP32, "FP32");
+    test_fp32_operands(Shtensor::Precision::BF16X3, "BF16X3");
     // an expression error surfaces as std::runtime_error with the reference's text
     bool threw = false;
     try {
