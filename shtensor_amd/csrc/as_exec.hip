@@ -677,7 +677,7 @@ __device__ inline unsigned short bf16_bits(float x)
 typedef float v2f_s __attribute__((ext_vector_type(2)));
 typedef __bf16 v2bf_s __attribute__((ext_vector_type(2)));
 typedef unsigned int v4u_s __attribute__((ext_vector_type(4)));
-__device__ inline void split3x8(const float (&x)[8], v8s& h, v8s& m, v8s& l)
+__device__ inline __attribute__((unused)) void split3x8(const float (&x)[8], v8s& h, v8s& m, v8s& l)
 {
   v4u_s hw, mw, lw;
 #pragma unroll
@@ -697,7 +697,7 @@ __device__ inline void split3x8(const float (&x)[8], v8s& h, v8s& m, v8s& l)
   l = __builtin_bit_cast(v8s, lw);
 }
 
-__device__ inline void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l)
+__device__ inline __attribute__((unused)) void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l)
 {
   h = bf16_bits(x);
   const float r1 = x - __uint_as_float(static_cast<unsigned int>(h) << 16);
