@@ -244,7 +244,9 @@ def cpu_info():
 def draw_requests(sht, ctx, n, budget, gen_cap, seed=1004):
     """Request mix of the serving config: order 2-4 (last mode of A
     contracted with B[k,l]), mode length L in [128, 1024], uniform block size
-    in {8,16,24,32}, occupancy p ~ U(0.05, 0.5), FP64 or FP32.  A draw whose
+    in {8,16,24,32}, occupancy p ~ U(0.05, 0.5), FP64 or FP32 (an FP32
+    request holds FP32 operand tensors, as a caller keeping FP32 operands
+    resident for that form would; C is FP64 either way).  A draw whose
     expected footprint exceeds the budget is rejected by the server's own
     rule; a draw that would take the generated operands past gen_cap is
     skipped (counted): the operands of the stream must fit in memory beside
@@ -261,7 +263,8 @@ def draw_requests(sht, ctx, n, budget, gen_cap, seed=1004):
         a_el = p * nb ** d * b ** d
         b_el = p * nb ** 2 * b ** 2
         c_el = min(1.0, p * nb * p) * nb ** d * b ** d  # C block present if any k matches
-        foot = 8.0 * (a_el + b_el + c_el)
+        ob = 4.0 if prec == sht.PREC_FP32 else 8.0  # operand bytes per element
+        foot = ob * (a_el + b_el) + 8.0 * c_el
         if foot > budget:
             rejected += 1
             continue
@@ -273,8 +276,9 @@ def draw_requests(sht, ctx, n, budget, gen_cap, seed=1004):
         ea, rest = expr.split(",")
         eb, ec = rest.split("->")
         bs = {ch: [b] * nb for ch in set(expr) if ch.isalpha()}
-        A = sht.Tensor(ctx, [bs[x] for x in ea])
-        B = sht.Tensor(ctx, [bs[x] for x in eb])
+        dt = np.float32 if prec == sht.PREC_FP32 else np.float64
+        A = sht.Tensor(ctx, [bs[x] for x in ea], dtype=dt)
+        B = sht.Tensor(ctx, [bs[x] for x in eb], dtype=dt)
         A.reserve_random(2000 + r, 0, p)
         B.reserve_random(2000 + r, 1, p)
         A.fill_random(2000 + r, 0, 0.0)

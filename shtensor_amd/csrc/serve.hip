@@ -159,7 +159,8 @@ constexpr int kPlanners = 2;      // planner threads
 // at most this long.
 constexpr double kBatchWindowMs = 2.0;
 
-size_t tensor_bytes(const sht_tensor* t) { return static_cast<size_t>(std::max<int64_t>(t->nelem, 0)) * 8; }
+// (operands may be FP32 tensors for the FP32 and split-BF16 forms)
+size_t tensor_bytes(const sht_tensor* t) { return static_cast<size_t>(std::max<int64_t>(t->nelem, 0)) * t->elem_size(); }
 
 // Arena bytes a plan holds (task list, records, tiles, schedules, staging).
 size_t plan_bytes(const sht_plan* P)

@@ -206,3 +206,28 @@ def test_served_fp32_operands_equal_alone(ctx, form):
     assert st["done"] == len(reqs)
     for ref, dC in zip(alone, outs):
         assert np.array_equal(dC.download(), ref)
+
+
+def test_served_fp32_operands_counted_at_four_bytes(ctx):
+    """The admission footprint counts FP32 operand tensors at 4 bytes per
+    element: a budget that holds A and B in FP32 but not in FP64 admits the
+    request, and its result equals the same contraction run alone."""
+    rng = np.random.default_rng(91)
+    sizes = {"i": [24, 16, 20], "k": [32, 8, 16], "j": [12, 28]}
+    A = random_bst(rng, [sizes["i"], sizes["k"]], 0.8)
+    B = random_bst(rng, [sizes["k"], sizes["j"]], 0.8)
+    dA, dB = to_device32(ctx, A), to_device32(ctx, B)
+    op_bytes = 4 * (dA.nelem() + dB.nelem())
+    bsC = [sizes["i"], sizes["j"]]
+    dC = sht.Tensor(ctx, bsC)
+    plan = sht.Plan(ctx, "ik,kj->ij", dA, dB, dC, sht.PREC_FP32)
+    plan.execute(1.0, 0.0)
+    sht.call("sht_device_synchronize")
+    plan.close()
+    alone = dC.download()
+    srv = sht.Server(ctx, int(1.5 * op_bytes))  # below 8 bytes per operand element
+    dC2 = sht.Tensor(ctx, bsC)
+    rid = srv.submit("ik,kj->ij", 1.0, dA, dB, 0.0, dC2, sht.PREC_FP32)
+    srv.wait(rid)
+    srv.close()
+    assert np.array_equal(dC2.download(), alone)
