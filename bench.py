@@ -459,6 +459,11 @@ def leg_c3(sht, orc, stream):
             rec = {}
             ops32 = (A.converted(np.float32, stream), B.converted(np.float32, stream))
             stream.synchronize()
+
+            def decades(T):
+                n = T.block_norms()
+                n = n[n > 0]
+                return round(float(np.log10(n.max() / n.min())), 2) if len(n) else 0.0
             for name, pr in (("fp64", sht.PREC_FP64), ("fp32", sht.PREC_FP32), ("bf16x3", sht.PREC_BF16X3),
                              ("fp32_f64in", sht.PREC_FP32), ("bf16x3_f64in", sht.PREC_BF16X3)):
                 C = sht.Tensor(ctx, Cc.block_sizes)
@@ -484,6 +489,16 @@ def leg_c3(sht, orc, stream):
                 C.close()
             for name in rec:
                 rec[name]["ratio_vs_fp64"] = round(rec[name]["gflops"] / rec["fp64"]["gflops"], 3)
+            # the span of the Frobenius block norms (log10 max/min over the
+            # present blocks): the decay applies to A over (i, k) and to B
+            # over (k, j), so C's blocks span about both
+            Cn = sht.Tensor(ctx, Cc.block_sizes)
+            cp = sht.Plan(ctx, expr, A, B, Cn, sht.PREC_FP64)
+            cp.execute(1.0, 0.0, stream)
+            stream.synchronize()
+            rec["norm_decades"] = {"A": decades(A), "B": decades(B), "C": decades(Cn)}
+            cp.close()
+            Cn.close()
             out[cfg] = rec
             log("%s leg: %s" % (cfg, rec))
         finally:
