@@ -234,6 +234,16 @@ sht_status sht_plan_autotune_ex(sht_plan* p, double alpha, double beta, int reps
 sht_status sht_plan_set_variant(sht_plan* p, int variant);
 sht_status sht_plan_get_variant(const sht_plan* p, int* variant);
 const char* sht_cs_variant_name(int variant);
+/* FP32 staging of FLOAT64 operands (FP32 and BF16X3 plans; no counterpart in
+ * the reference, which has no reduced-precision form): with 1, each
+ * C-stationary execution first converts A and B once to FP32 copies and the
+ * tiles read those; with 0, the tiles read and round the FP64 values at every
+ * use.  A new plan stages when its tiles read each operand element at least
+ * ten times on average and the arena holds the copies (4 bytes per element).
+ * Both forms round every operand to FP32 first: results are bitwise the same
+ * either way.  SHT_ERR_OOM when the arena cannot hold the copies.            */
+sht_status sht_plan_set_fp32_staging(sht_plan* p, int on);
+sht_status sht_plan_get_fp32_staging(const sht_plan* p, int* on);
 /* capture one execution into a hipGraph and replay it `replays` times       */
 sht_status sht_plan_graph_replay(sht_plan* p, double alpha, double beta, int replays, void* stream);
 /* task list (host copies): positions into A, B, C and the contracted-block

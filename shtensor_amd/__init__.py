@@ -411,6 +411,17 @@ class Plan:
         call("sht_plan_get_variant", C.c_void_p(self.ptr), C.byref(v))
         return v.value
 
+    def set_fp32_staging(self, on):
+        """FP32 / split-BF16 plans over FP64 operands: convert A and B to
+        FP32 once per execution (True) or round them at every use (False);
+        bitwise the same."""
+        call("sht_plan_set_fp32_staging", C.c_void_p(self.ptr), 1 if on else 0)
+
+    def get_fp32_staging(self):
+        v = C.c_int()
+        call("sht_plan_get_fp32_staging", C.c_void_p(self.ptr), C.byref(v))
+        return bool(v.value)
+
     def build_ms(self):
         """Host wall time of the build steps (ms): task list + C pattern,
         C-stationary tiles, A-stationary schedule (-1 if not built)."""

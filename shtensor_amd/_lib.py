@@ -53,6 +53,8 @@ SIGNATURES = {
     "sht_plan_autotune_ex": [_P, _D, _D, _I, _P, _I, _PI, _PI, _PI, C.POINTER(C.c_float)],
     "sht_plan_set_variant": [_P, _I],
     "sht_plan_get_variant": [_P, _PI],
+    "sht_plan_set_fp32_staging": [_P, _I],
+    "sht_plan_get_fp32_staging": [_P, _PI],
     "sht_server_counters": [_P, _PI64, _PI64, C.POINTER(_SZ)],
     "sht_kernel_info": [_P, C.c_char_p, _SZ],
     "sht_kernel_destroy": [_P],

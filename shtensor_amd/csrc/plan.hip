@@ -557,6 +557,68 @@ void build_schedule(sht_plan* P)
   build_tiles(P);
 }
 
+// Operand element reads of the C-stationary tiles (a tile reads its rows of
+// every A block and its columns of every B block of its C block's tasks;
+// 64-wide tiles assumed), over the elements A and B hold.
+double cs_reuse(const sht_plan* P)
+{
+  double reads = 0.0;
+  for (int64_t c = 0; c < P->ncb; ++c) {
+    const CBlkRec& r = P->h_cblk[c];
+    const double wk = static_cast<double>(P->h_work[c]);
+    reads += wk * (static_cast<double>(r.m) * ((r.n + 63) / 64) + static_cast<double>(r.n) * ((r.m + 63) / 64));
+  }
+  const double held = static_cast<double>(P->A->nelem) + static_cast<double>(P->B->nelem);
+  return held > 0.0 ? reads / held : 0.0;
+}
+
+void drop_stage32(sht_plan* P)
+{
+  Arena& arena = P->ctx->arena;
+  if (P->d_A32) arena.release(P->d_A32);
+  if (P->d_B32) arena.release(P->d_B32);
+  P->d_A32 = P->d_B32 = nullptr;
+  P->stage32 = false;
+}
+
+// The FP32 staging buffers of A and B; false (and nothing kept) when the
+// arena cannot hold them.
+bool make_stage32(sht_plan* P)
+{
+  if (P->stage32) return true;
+  Arena& arena = P->ctx->arena;
+  void* a = nullptr;
+  try {
+    a = arena.allocate(static_cast<size_t>(std::max<int64_t>(P->A->nelem, 1)) * 4 + kReadSlack);
+    P->d_B32 = arena.allocate(static_cast<size_t>(std::max<int64_t>(P->B->nelem, 1)) * 4 + kReadSlack);
+  } catch (const Error& e) {
+    if (e.code != SHT_ERR_OOM) throw;
+    if (a) arena.release(a);
+    return false;
+  }
+  P->d_A32 = a;
+  P->stage32 = true;
+  return true;
+}
+
+// A new FP32 or split-BF16 plan over FP64 operands stages them in FP32 when
+// its C-stationary tiles read each operand element at least kStage32Reuse
+// times on average (the conversion then costs less than reading FP64 values
+// at every use; SHT_FP32_STAGING=0 or 1 overrides) and the arena holds the
+// copies.  Both forms round every operand to FP32 first, so results are
+// bitwise the same with or without the staging.
+constexpr double kStage32Reuse = 10.0;
+void build_stage32(sht_plan* P)
+{
+  if ((P->prec != SHT_PREC_FP32 && P->prec != SHT_PREC_BF16X3) || P->A->dtype != SHT_FLOAT64 || P->ntasks == 0)
+    return;
+  static const int force = [] {
+    const char* e = std::getenv("SHT_FP32_STAGING");
+    return e ? std::atoi(e) : -1;
+  }();
+  if (force == 1 || (force < 0 && cs_reuse(P) >= kStage32Reuse)) make_stage32(P);
+}
+
 // C-stationary tile lists of the plan's current variant (P->cs_variant):
 // 0 automatic (below), 1 without the 128-row tiles, 2 with 128-row tiles in
 // any launch (FP64 / FP32), 3 tiles by descending cost (no XCD dealing),
@@ -1428,6 +1490,14 @@ void enqueue(sht_plan* P, double alpha, double beta, hipStream_t s, std::vector<
   a.beta_zero = bz ? 1 : 0;
   a.alpha_one = alpha == 1.0 ? 1 : 0;
   a.f32 = P->A->dtype == SHT_FLOAT32 ? 1 : 0;
+  if (P->stage32 && !az) {
+    // FP64 operands read many times: converted once, the tiles read FP32
+    launch_convert(Ad, true, P->d_A32, false, P->A->nelem, s);
+    launch_convert(Bd, true, P->d_B32, false, P->B->nelem, s);
+    a.A = P->d_A32;
+    a.B = P->d_B32;
+    a.f32 = 1;
+  }
   // Tile classes are independent (every C element belongs to one tile of one
   // class), so with several classes they run concurrently: the caller's
   // stream forks to up to three plan-owned streams and joins them again, so
@@ -1503,6 +1573,7 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
       sht::build_tasks(P.get());
       const double t1 = sht::wall_ms();
       sht::build_schedule(P.get());
+      sht::build_stage32(P.get());
       P->ms_cstat = sht::wall_ms() - t1;
       P->ms_tasks = t1 - t0;
       sht::setup_staging(P.get());
@@ -1763,6 +1834,27 @@ sht_status sht_plan_get_variant(const sht_plan* P, int* variant)
   return guard([&] {
     SHT_REQUIRE(P && variant, SHT_ERR_INVALID, "null argument");
     *variant = P->cs_variant;
+  });
+}
+
+sht_status sht_plan_set_fp32_staging(sht_plan* P, int on)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && (on == 0 || on == 1), SHT_ERR_INVALID, "staging must be 0 or 1");
+    SHT_REQUIRE(!on || ((P->prec == SHT_PREC_FP32 || P->prec == SHT_PREC_BF16X3) && P->A->dtype == SHT_FLOAT64),
+                SHT_ERR_INVALID, "FP32 staging needs FLOAT64 operands and the FP32 or BF16X3 form");
+    if ((on != 0) == P->stage32) return;
+    drop_graph(P);
+    if (!on) sht::drop_stage32(P);
+    else SHT_REQUIRE(sht::make_stage32(P), SHT_ERR_OOM, "Out of memory");
+  });
+}
+
+sht_status sht_plan_get_fp32_staging(const sht_plan* P, int* on)
+{
+  return guard([&] {
+    SHT_REQUIRE(P && on, SHT_ERR_INVALID, "null argument");
+    *on = P->stage32 ? 1 : 0;
   });
 }
 

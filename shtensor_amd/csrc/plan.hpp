@@ -222,6 +222,12 @@ struct sht_plan {
   int64_t* d_tc = nullptr;
   int64_t* d_kk = nullptr;
   sht::TaskRec* d_tasks = nullptr;
+  // FP32 staging of FP64 operands for the C-stationary FP32 and split-BF16
+  // classes: each execution converts A and B once and the tiles read the
+  // FP32 copies (chosen when the tiles read each operand element many times)
+  bool stage32 = false;
+  void* d_A32 = nullptr;
+  void* d_B32 = nullptr;
   sht::CBlkRec* d_cblk = nullptr;
   sht::TileRec* d_tiles = nullptr;
   struct Launch {
@@ -288,7 +294,7 @@ struct sht_plan {
                     static_cast<void*>(d_as_ccol), static_cast<void*>(d_as_unit),
                     static_cast<void*>(d_as_unitw), static_cast<void*>(d_as_sweep4),
                     static_cast<void*>(d_as_unit4), static_cast<void*>(d_bt_rec), d_bt,
-                    static_cast<void*>(d_as_zero), static_cast<void*>(d_as_qctr)})
+                    static_cast<void*>(d_as_zero), static_cast<void*>(d_as_qctr), d_A32, d_B32})
       if (p) a.release(p);
     if (graph) (void)hipGraphExecDestroy(graph);
     if (graph_def) (void)hipGraphDestroy(graph_def);
