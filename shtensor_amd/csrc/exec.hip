@@ -545,12 +545,20 @@ __device__ inline void split3x2_cs(float x0, float x1, uint32_t& h, uint32_t& m,
 // 64 x 16 for at most 16).  Each 32-k chunk of A (TR rows x 32) and B (32 x
 // TC columns) is loaded once by the workgroup (lane: one row or column,
 // TR/8 or TC/8 consecutive k), split into three bf16 planes as it is
-// written to LDS (two stages, one barrier per chunk; rows padded to 40
-// bf16) and read by the waves as 16-byte MFMA fragments; the waves split
+// written to LDS (two stages, one barrier per chunk; 32-element rows with
+// XOR-rotated 16-byte pieces, w3k) and read by the waves as 16-byte MFMA
+// fragments; the waves split
 // the tile into 2 x 2 (or 4 x 1 for 16 columns) parts of 16 x 16 MFMA
 // tiles.  Loads are unconditional (clamped in-bounds addresses); k >= K is
 // zeroed before the split.
-constexpr int kW3Pitch = 40;
+constexpr int kW3Pitch = 32;
+// LDS position of contracted index k (0..31) in a plane row: the row's four
+// 16-byte pieces are XOR-rotated by (row / 2) % 4, so the loaders' stores
+// (consecutive rows, one piece) and the waves' 16-byte fragment reads (16
+// consecutive rows, four pieces) meet no bank conflicts without row padding
+// (unpadded rows keep the 64 x 64 tile at 48 KB of LDS: three workgroups
+// per CU instead of two).
+__device__ inline int w3k(int row, int k) { return (((k >> 3) ^ ((row >> 1) & 3)) << 3) | (k & 7); }
 template <int TR, int TC, class TE = double>
 __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
 {
@@ -628,9 +636,9 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       m[e / 2] = y;                                                                        \
       l[e / 2] = z;                                                                        \
     }                                                                                      \
-    *reinterpret_cast<vas*>(&sA[stg][0][ra_][KA * ga]) = h;                                \
-    *reinterpret_cast<vas*>(&sA[stg][1][ra_][KA * ga]) = m;                                \
-    *reinterpret_cast<vas*>(&sA[stg][2][ra_][KA * ga]) = l;                                \
+    *reinterpret_cast<vas*>(&sA[stg][0][ra_][w3k(ra_, KA * ga)]) = h;                                \
+    *reinterpret_cast<vas*>(&sA[stg][1][ra_][w3k(ra_, KA * ga)]) = m;                                \
+    *reinterpret_cast<vas*>(&sA[stg][2][ra_][w3k(ra_, KA * ga)]) = l;                                \
     vbs hb, mb, lb;                                                                        \
     _Pragma("unroll") for (int e = 0; e < KB; e += 2) {                                    \
       uint32_t x, y, z;                                                                    \
@@ -640,9 +648,9 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       mb[e / 2] = y;                                                                       \
       lb[e / 2] = z;                                                                       \
     }                                                                                      \
-    *reinterpret_cast<vbs*>(&sB[stg][0][cb_][KB * gb]) = hb;                               \
-    *reinterpret_cast<vbs*>(&sB[stg][1][cb_][KB * gb]) = mb;                               \
-    *reinterpret_cast<vbs*>(&sB[stg][2][cb_][KB * gb]) = lb;                               \
+    *reinterpret_cast<vbs*>(&sB[stg][0][cb_][w3k(cb_, KB * gb)]) = hb;                               \
+    *reinterpret_cast<vbs*>(&sB[stg][1][cb_][w3k(cb_, KB * gb)]) = mb;                               \
+    *reinterpret_cast<vbs*>(&sB[stg][2][cb_][w3k(cb_, KB * gb)]) = lb;                               \
   }
     v4f part[TRt][TCt];
 #pragma unroll
@@ -661,15 +669,15 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
         v8s ah[TRt], am[TRt], al[TRt], bh[TCt], bm[TCt], bl[TCt];
 #pragma unroll
         for (int rt = 0; rt < TRt; ++rt) {
-          ah[rt] = *reinterpret_cast<const v8s*>(&sA[stg][0][wr + 16 * rt + li][8 * lk]);
-          am[rt] = *reinterpret_cast<const v8s*>(&sA[stg][1][wr + 16 * rt + li][8 * lk]);
-          al[rt] = *reinterpret_cast<const v8s*>(&sA[stg][2][wr + 16 * rt + li][8 * lk]);
+          ah[rt] = *reinterpret_cast<const v8s*>(&sA[stg][0][wr + 16 * rt + li][w3k(li, 8 * lk)]);
+          am[rt] = *reinterpret_cast<const v8s*>(&sA[stg][1][wr + 16 * rt + li][w3k(li, 8 * lk)]);
+          al[rt] = *reinterpret_cast<const v8s*>(&sA[stg][2][wr + 16 * rt + li][w3k(li, 8 * lk)]);
         }
 #pragma unroll
         for (int ct = 0; ct < TCt; ++ct) {
-          bh[ct] = *reinterpret_cast<const v8s*>(&sB[stg][0][wc + 16 * ct + li][8 * lk]);
-          bm[ct] = *reinterpret_cast<const v8s*>(&sB[stg][1][wc + 16 * ct + li][8 * lk]);
-          bl[ct] = *reinterpret_cast<const v8s*>(&sB[stg][2][wc + 16 * ct + li][8 * lk]);
+          bh[ct] = *reinterpret_cast<const v8s*>(&sB[stg][0][wc + 16 * ct + li][w3k(li, 8 * lk)]);
+          bm[ct] = *reinterpret_cast<const v8s*>(&sB[stg][1][wc + 16 * ct + li][w3k(li, 8 * lk)]);
+          bl[ct] = *reinterpret_cast<const v8s*>(&sB[stg][2][wc + 16 * ct + li][w3k(li, 8 * lk)]);
         }
 #pragma unroll
         for (int rt = 0; rt < TRt; ++rt)
