@@ -51,6 +51,7 @@ KERNEL_SYMBOL = {
     "mfma64_n32w": "k_mfmaw<0, 64, 32", "mfma32_n32w": "k_mfmaw<1, 64, 32",
     "mfma32_n16w_r128": "k_mfmaw<1, 128, 16", "mfma32_n32w_r128": "k_mfmaw<1, 128, 32", "bf16x3_n32s": "k_bf3w<32, 32",
     "bf16x3_n16w": "k_bf3w<64, 16",
+    "mfma64_persist": "k_csp<0", "mfma32_persist": "k_csp<1", "bf16x3_persist": "k_bf3q<",
     "mfma64_n16_r128": "k_mfma64<1, 8", "mfma64_n32_r128": "k_mfma64<2, 8",
 }
 CHOICES = [5, 7, 9, 11, 13, 15, 21, 28]

@@ -230,7 +230,9 @@ sht_status sht_plan_autotune_ex(sht_plan* p, double alpha, double beta, int reps
  * tiles in large FP64/FP32 launches, XCD-aware order), 1 without 128-row
  * tiles, 2 with 128-row tiles in any launch, 3 tiles by descending cost,
  * 4 uniform 64 x 64 four-wave tiles for every block (128 x 32 for the fma
- * chain): a fixed-tile reference.  Results do not depend on it.             */
+ * chain): a fixed-tile reference, 5 one persistent launch serving every tile
+ * of the plan (four-wave classes; not for SHT_PREC_FP64_EXACT).  Results do
+ * not depend on it.                                                         */
 sht_status sht_plan_set_variant(sht_plan* p, int variant);
 sht_status sht_plan_get_variant(const sht_plan* p, int* variant);
 const char* sht_cs_variant_name(int variant);

@@ -1204,11 +1204,16 @@ __global__ void k_clear_qctr(unsigned long long* c)
   if (threadIdx.x < 8) c[threadIdx.x] = 0;
 }
 
-// Grid of the persistent grouped kernel f: the workgroups the device keeps
-// resident (a multiple of eight, one set per XCD queue), at most one per
-// workgroup task.  The occupancy is queried once per device and kernel.
+// Grid of the persistent grouped kernel f (one workgroup task per work item).
 template <class Kern>
 unsigned persistent_grid(Kern f, int threads, int64_t ntasks)
+{
+  return resident_grid(reinterpret_cast<const void*>(f), threads, ntasks);
+}
+
+}  // namespace
+
+unsigned resident_grid(const void* f, int threads, int64_t nwork)
 {
   static std::mutex mu;
   static std::map<std::pair<int, const void*>, int> cache;
@@ -1217,7 +1222,7 @@ unsigned persistent_grid(Kern f, int threads, int64_t ntasks)
   int resident = 0;
   {
     std::lock_guard<std::mutex> lk(mu);
-    auto key = std::make_pair(dev, reinterpret_cast<const void*>(f));
+    auto key = std::make_pair(dev, f);
     auto it = cache.find(key);
     if (it == cache.end()) {
       int per_cu = 0, cus = 0;
@@ -1227,11 +1232,9 @@ unsigned persistent_grid(Kern f, int threads, int64_t ntasks)
     }
     resident = it->second;
   }
-  const int64_t want = (ntasks + 7) / 8 * 8;
+  const int64_t want = (nwork + 7) / 8 * 8;
   return static_cast<unsigned>(std::min<int64_t>(want, resident));
 }
-
-}  // namespace
 
 void launch_as(AsClass k, const AsArgs& a, hipStream_t s)
 {

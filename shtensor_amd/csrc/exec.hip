@@ -256,8 +256,26 @@ __global__ __launch_bounds__(64) void k_mfma64(ExecArgs p)
 // after task in task order and k ascending, so every C element sees the
 // same chain as in the one-wave kernels of the same precision.
 // ---------------------------------------------------------------------------
-template <int F, int TR, int TC, class TE = double>
-__global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
+// The LDS stages of one TR x TC tile.  Row padding against bank conflicts of
+// the loaders' stores and the waves' fragment reads, per tile shape
+// (measured on C3 b=32, 32 x 32 tiles: FP64 0.315-0.33 -> 0.26-0.31 ms, FP32
+// 0.221 -> 0.194 ms on one box and unchanged on another; C0's 64 x 16 FP64
+// tiles 0.0426 -> 0.0407 ms; no gain measured for the other shapes, which
+// keep none).
+template <int F, int TR, int TC>
+struct MfmawLds {
+  typedef typename std::conditional<F == 0, double, float>::type TL;
+  static constexpr bool sq32 = TR == 32 && TC == 32;
+  static constexpr int PA = F == 0 ? (sq32 ? 1 : 0) : (sq32 ? 16 : 0);
+  static constexpr int PB = F == 0 ? (sq32 ? 1 : (TC == 16 ? 1 : 0)) : (sq32 ? 18 : 0);
+  TL sA[2][16][TR + PA];  // [stage][k][row of the tile]
+  TL sB[2][16][TC + PB];  // [stage][k][column of the tile]
+};
+
+// One tile (called by the whole workgroup; the tile is not a padding tile).
+template <int F, int TR, int TC, class TE>
+__device__ __attribute__((always_inline)) inline void mfmaw_tile(const ExecArgs& p, const TileRec& tile,
+                                                                 MfmawLds<F, TR, TC>& L)
 {
   static_assert(F != 0 || std::is_same<TE, double>::value, "FP64 MFMA reads FP64 operands");
   // waves: WR along the rows x WC along the columns of the tile
@@ -265,9 +283,7 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   constexpr int TRt = TR / WR / 16, TCt = TC / WC / 16;  // 16 x 16 MFMA tiles per wave
   constexpr int EA = TR / 16, EB = TC / 16;  // elements of A / of B per loader lane and chunk
   static_assert(TRt >= 1 && TCt >= 1 && EA % 2 == 0 && (EB == 1 || EB % 2 == 0), "tile shape");
-  typedef typename std::conditional<F == 0, double, float>::type TL;
-  const TileRec tile = p.tiles[blockIdx.x];
-  if (tile.pad) return;  // (uniform over the workgroup)
+  typedef typename MfmawLds<F, TR, TC>::TL TL;
   const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -276,16 +292,8 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
   const TE* A = static_cast<const TE*>(p.A);
   const TE* B = static_cast<const TE*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  // Row padding of the LDS stages against bank conflicts of the loaders'
-  // stores and the waves' fragment reads, per tile shape (measured on C3
-  // b=32, 32 x 32 tiles: FP64 0.315-0.33 -> 0.26-0.31 ms, FP32 0.221 -> 0.194
-  // ms on one box and unchanged on another; C0's 64 x 16 FP64 tiles 0.0426 ->
-  // 0.0407 ms; no gain measured for the other shapes, which keep none).
-  constexpr bool sq32 = TR == 32 && TC == 32;
-  constexpr int PA = F == 0 ? (sq32 ? 1 : 0) : (sq32 ? 16 : 0);
-  constexpr int PB = F == 0 ? (sq32 ? 1 : (TC == 16 ? 1 : 0)) : (sq32 ? 18 : 0);
-  __shared__ TL sA[2][16][TR + PA];  // [stage][k][row of the tile]
-  __shared__ TL sB[2][16][TC + PB];  // [stage][k][column of the tile]
+  auto& sA = L.sA;
+  auto& sB = L.sB;
   // column of accumulator element v (the f64 and f32 MFMA results differ)
   auto colof = [&](int ct, int v) { return tile.n0 + wc + 16 * ct + (F == 0 ? 4 * v + lk : 4 * lk + v); };
   v4d acc[TRt][TCt];
@@ -413,6 +421,15 @@ __global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
         const int j = colof(ct, v);
         if (i < M && j < N) Cb[i + static_cast<int64_t>(j) * M] = acc[rt][ct][v];
       }
+}
+
+template <int F, int TR, int TC, class TE = double>
+__global__ __launch_bounds__(256) void k_mfmaw(ExecArgs p)
+{
+  __shared__ MfmawLds<F, TR, TC> L;
+  const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // (uniform over the workgroup)
+  mfmaw_tile<F, TR, TC, TE>(p, tile, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,16 +576,22 @@ constexpr int kW3Pitch = 32;
 // (unpadded rows keep the 64 x 64 tile at 48 KB of LDS: three workgroups
 // per CU instead of two).
 __device__ inline int w3k(int row, int k) { return (((k >> 3) ^ ((row >> 1) & 3)) << 3) | (k & 7); }
-template <int TR, int TC, class TE = double>
-__global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
+template <int TR, int TC>
+struct Bf3Lds {
+  short sA[2][3][TR][kW3Pitch];  // [stage][plane][row of the tile][k]
+  short sB[2][3][TC][kW3Pitch];  // [stage][plane][column of the tile][k]
+};
+
+// One tile (called by the whole workgroup; the tile is not a padding tile).
+template <int TR, int TC, class TE>
+__device__ __attribute__((always_inline)) inline void bf3w_tile(const ExecArgs& p, const TileRec& tile,
+                                                                Bf3Lds<TR, TC>& L)
 {
   // waves: WR along the rows x WC along the columns of the tile
   constexpr int WC = TC >= 32 ? 2 : 1, WR = 4 / WC;
   constexpr int TRt = TR / WR / 16, TCt = TC / WC / 16;  // 16 x 16 MFMA tiles per wave
   constexpr int KA = TR / 8, KB = TC / 8;  // contracted indices per loader lane and chunk
   static_assert(TRt >= 1 && TCt >= 1 && KB >= 2, "tile shape");
-  const TileRec tile = p.tiles[blockIdx.x];
-  if (tile.pad) return;  // (uniform over the workgroup)
   const CBlkRec cb{tile.c_off, tile.m, tile.n, tile.t0, tile.t1};  // (carried by the tile record)
   const int M = cb.m, N = cb.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -577,8 +600,8 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
   const TE* A = static_cast<const TE*>(p.A);
   const TE* B = static_cast<const TE*>(p.B);
   double* Cb = static_cast<double*>(p.C) + cb.c_off;
-  __shared__ short sA[2][3][TR][kW3Pitch];  // [stage][plane][row of the tile][k]
-  __shared__ short sB[2][3][TC][kW3Pitch];  // [stage][plane][column of the tile][k]
+  auto& sA = L.sA;
+  auto& sB = L.sB;
   v4d acc[TRt][TCt];
 #pragma unroll
   for (int rt = 0; rt < TRt; ++rt)
@@ -722,6 +745,84 @@ __global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
       }
 }
 
+template <int TR, int TC, class TE = double>
+__global__ __launch_bounds__(256) void k_bf3w(ExecArgs p)
+{
+  __shared__ Bf3Lds<TR, TC> L;
+  const TileRec tile = p.tiles[blockIdx.x];
+  if (tile.pad) return;  // (uniform over the workgroup)
+  bf3w_tile<TR, TC, TE>(p, tile, L);
+}
+
+// ---------------------------------------------------------------------------
+// The persistent C-stationary launch: one grid of the workgroups the device
+// keeps resident (a multiple of eight), serving every tile of the plan (its
+// FP64, FP32 or split-BF16 four-wave classes mixed, each tile's class in its
+// record).  Workgroup b serves XCD queue b % 8 (queue x holds tiles x, x + 8,
+// x + 16, ...): with W workgroups per queue, it takes the queue's positions
+// b / 8, b / 8 + W, b / 8 + 2W, ... (a fixed deal: claiming positions from an
+// atomic counter per queue made C0 4.6x and C2 1.3x slower than the
+// per-class launches, every claim a round trip to one contended address).
+// Each tile runs the code of its class's own kernel (mfmaw_tile / bf3w_tile)
+// in LDS shared by the classes; a barrier separates consecutive tiles.
+// ---------------------------------------------------------------------------
+__device__ inline bool next_tile(const ExecArgs& p, int64_t& t, TileRec& tile)
+{
+  const int64_t x = blockIdx.x & 7u, w = gridDim.x >> 3;
+  for (; t < p.qlen; t += w) {
+    const int64_t ti = 8 * t + x;
+    if (ti >= p.ntiles) continue;  // (the last round of the queues is short)
+    tile = p.tiles[ti];
+    if (!tile.pad) {
+      t += w;
+      __syncthreads();  // every wave is done with the previous tile's LDS
+      return true;
+    }
+  }
+  return false;
+}
+
+template <int F, class TE>
+__global__ __launch_bounds__(256) void k_csp(ExecArgs p)
+{
+  __shared__ union {
+    MfmawLds<F, 64, 64> w;
+    MfmawLds<F, 32, 32> s;
+    MfmawLds<F, 64, 16> n;
+    MfmawLds<F, 64, 32> m;
+  } L;
+  TileRec tile;
+  int64_t t = blockIdx.x >> 3;
+  while (next_tile(p, t, tile)) {
+    switch (static_cast<KClass>(tile.kc)) {
+      case (F == 0 ? KClass::Mfma64W : KClass::F32W): mfmaw_tile<F, 64, 64, TE>(p, tile, L.w); break;
+      case (F == 0 ? KClass::Mfma64S : KClass::F32S): mfmaw_tile<F, 32, 32, TE>(p, tile, L.s); break;
+      case (F == 0 ? KClass::Mfma64N : KClass::F32N): mfmaw_tile<F, 64, 16, TE>(p, tile, L.n); break;
+      default: mfmaw_tile<F, 64, 32, TE>(p, tile, L.m); break;  // Mfma64M / F32M
+    }
+  }
+}
+
+template <class TE>
+__global__ __launch_bounds__(256) void k_bf3q(ExecArgs p)
+{
+  __shared__ union {
+    Bf3Lds<64, 64> w;
+    Bf3Lds<32, 32> s;
+    Bf3Lds<64, 16> n;
+  } L;
+  TileRec tile;
+  int64_t t = blockIdx.x >> 3;
+  while (next_tile(p, t, tile)) {
+    switch (static_cast<KClass>(tile.kc)) {
+      case KClass::Bf16W: bf3w_tile<64, 64, TE>(p, tile, L.w); break;
+      case KClass::Bf16S: bf3w_tile<32, 32, TE>(p, tile, L.s); break;
+      default: bf3w_tile<64, 16, TE>(p, tile, L.n); break;  // Bf16N
+    }
+  }
+}
+
+
 // ---------------------------------------------------------------------------
 // block permutation (Utils::reshape per block)
 // ---------------------------------------------------------------------------
@@ -834,7 +935,7 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
   }
   SHT_REQUIRE(!a.f32 || k == KClass::F32x16h || k == KClass::F32W || k == KClass::F32N || k == KClass::F32M ||
                   k == KClass::F32Nq || k == KClass::F32Mq || k == KClass::F32S || k == KClass::Bf16W ||
-                  k == KClass::Bf16S || k == KClass::Bf16N,
+                  k == KClass::Bf16S || k == KClass::Bf16N || k == KClass::Persist32 || k == KClass::PersistBf16,
               SHT_ERR_STATE, "FP32 operands reach an FP64 kernel class");
   switch (k) {
     case KClass::Exact8: hipLaunchKernelGGL(k_exact<8>, g, b, 0, s, a); break;
@@ -859,6 +960,26 @@ void launch_exec(KClass k, const ExecArgs& a, hipStream_t s)
     case KClass::Bf16W: SHT_EXEC_F32(k_bf3w, 64, 64); break;
     case KClass::Bf16S: SHT_EXEC_F32(k_bf3w, 32, 32); break;
     case KClass::Bf16N: SHT_EXEC_F32(k_bf3w, 64, 16); break;
+    case KClass::Persist64:
+    case KClass::Persist32:
+    case KClass::PersistBf16: {
+      ExecArgs q = a;
+      q.qlen = (a.ntiles + 7) / 8;
+      const void* f = k == KClass::Persist64
+                          ? reinterpret_cast<const void*>(k_csp<0, double>)
+                          : (k == KClass::Persist32
+                                 ? (a.f32 ? reinterpret_cast<const void*>(k_csp<1, float>)
+                                          : reinterpret_cast<const void*>(k_csp<1, double>))
+                                 : (a.f32 ? reinterpret_cast<const void*>(k_bf3q<float>)
+                                          : reinterpret_cast<const void*>(k_bf3q<double>)));
+      const dim3 gp(resident_grid(f, 256, a.ntiles));
+      if (k == KClass::Persist64) hipLaunchKernelGGL((k_csp<0, double>), gp, b, 0, s, q);
+      else if (k == KClass::Persist32 && a.f32) hipLaunchKernelGGL((k_csp<1, float>), gp, b, 0, s, q);
+      else if (k == KClass::Persist32) hipLaunchKernelGGL((k_csp<1, double>), gp, b, 0, s, q);
+      else if (a.f32) hipLaunchKernelGGL(k_bf3q<float>, gp, b, 0, s, q);
+      else hipLaunchKernelGGL(k_bf3q<double>, gp, b, 0, s, q);
+      break;
+    }
     default: throw Error(SHT_ERR_INVALID, "unknown kernel class");
   }
 #undef SHT_EXEC_F32

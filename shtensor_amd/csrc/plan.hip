@@ -475,7 +475,7 @@ void build_tiles(sht_plan* P);
 // padding tile.
 struct TilePlace {
   int64_t pos;
-  int32_t cb, m0, n0, unused;
+  int32_t cb, m0, n0, kc;  // kc: the tile's class
 };
 
 __global__ void k_tile_pads(TileRec* t, int64_t n)
@@ -499,6 +499,7 @@ __global__ void k_tile_place(TileRec* t, const TilePlace* items, int64_t nitems,
   r.cb = p.cb;
   r.m0 = p.m0;
   r.n0 = p.n0;
+  r.kc = p.kc;
   const CBlkRec c = cblk[p.cb];
   r.c_off = c.c_off;
   r.m = c.m;
@@ -623,7 +624,9 @@ void build_stage32(sht_plan* P)
 // 0 automatic (below), 1 without the 128-row tiles, 2 with 128-row tiles in
 // any launch (FP64 / FP32), 3 tiles by descending cost (no XCD dealing),
 // 4 uniform 64 x 64 four-wave tiles for every block (128 x 32 for the fma
-// chain): the fixed-tile reference the autotuner is measured against.
+// chain): the fixed-tile reference the autotuner is measured against;
+// 5 every tile in one persistent launch (the four-wave FP64 / FP32 /
+// split-BF16 classes by block width, XCD queues as for the grouped classes).
 void build_tiles(sht_plan* P)
 {
   Arena& arena = P->ctx->arena;
@@ -674,8 +677,14 @@ void build_tiles(sht_plan* P)
   int64_t n64 = 0;
   for (int64_t c = 0; c < ncb; ++c) n64 += (cb[c].m + 63) / 64;
   const bool quad_ok = P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32;
-  const bool quad = quad_ok && (variant == 2 || (variant != 1 && variant != 4 && n64 > 64 * 1024));
+  // variant 5: one persistent launch over every tile (four-wave classes only)
+  const bool persist = variant == 5;
+  const bool quad = quad_ok && !persist && (variant == 2 || (variant != 1 && variant != 4 && n64 > 64 * 1024));
   const bool uniform = variant == 4;
+  const KClass persist_k = P->prec == SHT_PREC_FP64   ? KClass::Persist64
+                           : P->prec == SHT_PREC_FP32 ? KClass::Persist32
+                                                      : KClass::PersistBf16;
+  SHT_REQUIRE(!persist || !exact, SHT_ERR_INVALID, "the persistent variant needs the FP64, FP32 or BF16X3 form");
   // the tile class of C block c (its row tiles may take the 32- or 128-row
   // form of it)
   auto main_class = [&](int64_t c) {
@@ -688,15 +697,18 @@ void build_tiles(sht_plan* P)
     if (P->prec == SHT_PREC_FP32) return N <= 16 ? KClass::F32N : (N <= 32 ? KClass::F32M : KClass::F32W);
     if (P->prec == SHT_PREC_BF16X3) return N <= 16 ? KClass::Bf16N : (N <= 32 ? KClass::Bf16S : KClass::Bf16W);
     if (N > 32) return KClass::Mfma64W;
+    if (persist) return N <= 16 ? KClass::Mfma64N : KClass::Mfma64M;
     // (FP64 large launches: one-wave 64-row tiles beside the 128-row ones,
     // measured faster there than the four-wave 64-row forms)
     if (quad) return N <= 16 ? KClass::Mfma16 : KClass::Mfma32;
     return N <= 16 ? KClass::Mfma64N : KClass::Mfma64M;
   };
-  // Calls f(class, m0, n0, cost) for the tiles of C block c in order: row
-  // tiles of 128 rows while more than 96 remain (FP64 and FP32 up to 32
-  // columns, large launches), then 64; the last row tile of an MFMA class
-  // takes the 32-row form when it can.
+  // Calls f(launch, m0, n0, cost, class) for the tiles of C block c in
+  // order: row tiles of 128 rows while more than 96 remain (FP64 and FP32 up
+  // to 32 columns, large launches), then 64; the last row tile of an MFMA
+  // class takes the 32-row form when it can (the persistent variant: when
+  // that form is a four-wave class too).  The launch is the tile's class, or
+  // the persistent launch that serves every class.
   auto each_tile = [&](int64_t c, auto&& f) {
     const int M = cb[c].m, N = cb[c].n;
     const KClass k = main_class(c);
@@ -704,20 +716,20 @@ void build_tiles(sht_plan* P)
     for (int m0 = 0; m0 < M;) {
       KClass kt = k;
       if (uniform) kt = k;
-      else if (!exact && M - m0 <= 32) kt = half_rows(k);
+      else if (!exact && M - m0 <= 32 && (!persist || kclass_waves(half_rows(k)) == 4)) kt = half_rows(k);
       else if (quad && M - m0 > 96) kt = quad_rows(k);
       const int rows = kclass_rows(kt);
       for (int n0 = 0; n0 < N; n0 += cols)
-        f(static_cast<int>(kt), m0, n0,
-          static_cast<double>(std::min(rows, M - m0)) * std::min(cols, N - n0) * (w[c] + 1));
+        f(static_cast<int>(persist ? persist_k : kt), m0, n0,
+          static_cast<double>(std::min(rows, M - m0)) * std::min(cols, N - n0) * (w[c] + 1), static_cast<int>(kt));
       m0 += rows;
     }
   };
-  auto place = [](int64_t pos, int64_t c, int m0, int n0) {
-    return TilePlace{pos, static_cast<int32_t>(c), m0, n0, 0};
+  auto place = [](int64_t pos, int64_t c, int m0, int n0, int kt) {
+    return TilePlace{pos, static_cast<int32_t>(c), m0, n0, kt};
   };
   std::vector<int64_t> count(NK, 0);
-  for (int64_t c = 0; c < ncb; ++c) each_tile(c, [&](int k, int, int, double) { ++count[k]; });
+  for (int64_t c = 0; c < ncb; ++c) each_tile(c, [&](int k, int, int, double, int) { ++count[k]; });
   // Classes ordered by descending tile cost (variant 3, and small launches
   // when A and B are alike) and classes dealt to the XCD queues by group
   std::vector<char> by_cost(NK, 0);
@@ -727,14 +739,15 @@ void build_tiles(sht_plan* P)
     int64_t c;
     int32_t m0, n0;
     double cost;
+    int32_t kt;
   };
   std::vector<std::vector<CostTile>> ct(NK);
   for (int k = 0; k < NK; ++k)
     if (by_cost[k]) ct[k].reserve(static_cast<size_t>(count[k]));
   if (std::any_of(by_cost.begin(), by_cost.end(), [](char b) { return b != 0; }))
     for (int64_t c = 0; c < ncb; ++c)
-      each_tile(c, [&](int k, int m0, int n0, double cost) {
-        if (by_cost[k]) ct[k].push_back({c, m0, n0, cost});
+      each_tile(c, [&](int k, int m0, int n0, double cost, int kt) {
+        if (by_cost[k]) ct[k].push_back({c, m0, n0, cost, kt});
       });
   // The grouped classes: C blocks in group-key order (a stable counting sort;
   // keys are bounded by the number of C block rows or squares), so the tiles
@@ -761,7 +774,7 @@ void build_tiles(sht_plan* P)
   };
   std::vector<std::vector<Group>> grp(NK);
   for (int64_t c : corder)
-    each_tile(c, [&](int k, int, int, double cost) {
+    each_tile(c, [&](int k, int, int, double cost, int) {
       if (by_cost[k]) return;
       auto& g = grp[k];
       if (g.empty() || g.back().key != key[c]) g.push_back({key[c], 0, 0.0, 0, 0});
@@ -817,7 +830,7 @@ void build_tiles(sht_plan* P)
     std::sort(o.begin(), o.end());
     for (size_t i = 0; i < o.size(); ++i) {
       const CostTile& t = ct[k][o[i].second];
-      items.push_back(place(first[k] + static_cast<int64_t>(i), t.c, t.m0, t.n0));
+      items.push_back(place(first[k] + static_cast<int64_t>(i), t.c, t.m0, t.n0, t.kt));
     }
   }
   {
@@ -826,7 +839,7 @@ void build_tiles(sht_plan* P)
     std::vector<int64_t> gi(NK, -1), pos(NK, 0);
     std::vector<int64_t> gkey(NK, -1);
     for (int64_t c : corder)
-      each_tile(c, [&](int k, int m0, int n0, double) {
+      each_tile(c, [&](int k, int m0, int n0, double, int kt) {
         if (by_cost[k]) return;
         if (gkey[k] != key[c] || gi[k] < 0) {
           ++gi[k];
@@ -834,7 +847,7 @@ void build_tiles(sht_plan* P)
           pos[k] = 0;
         }
         const Group& g = grp[k][gi[k]];
-        items.push_back(place(first[k] + (g.slot0 + pos[k]++) * kXcd + g.x, c, m0, n0));
+        items.push_back(place(first[k] + (g.slot0 + pos[k]++) * kXcd + g.x, c, m0, n0, kt));
       });
   }
   if (total > 0) {
@@ -1642,7 +1655,8 @@ const char* sht_kclass_name(int k)
                                 "mfma32_n64w",      "bf16x3_n64w",      "mfma64_n32s",      "mfma32_n32s",
                                 "bf16x3_n32s",      "mfma64_n16_r128",  "mfma64_n32_r128",  "mfma64_n16w",
                                 "mfma32_n16w",      "bf16x3_n16w",      "mfma64_n32w",      "mfma32_n32w",
-                                "mfma32_n16w_r128", "mfma32_n32w_r128"};
+                                "mfma32_n16w_r128", "mfma32_n32w_r128", "mfma64_persist",   "mfma32_persist",
+                                "bf16x3_persist"};
   static_assert(sizeof(names) / sizeof(names[0]) == static_cast<size_t>(sht::KClass::Count),
                 "a name per C-stationary class");
   if (k < 0 || k >= static_cast<int>(sizeof(names) / sizeof(names[0]))) return "unknown";
@@ -1729,6 +1743,7 @@ void autotune(sht_plan* P, double alpha, double beta, int reps, hipStream_t s, s
   const bool quad_ok = P->prec == SHT_PREC_FP64 || P->prec == SHT_PREC_FP32;
   for (int v = 0; v <= 4; ++v)
     if (quad_ok || (v != 1 && v != 2)) cand.push_back({0, v});
+  if (P->prec != SHT_PREC_FP64_EXACT) cand.push_back({0, 5});
   if (sht::as_supported(P)) {
     cand.push_back({1, 0});
     // the fma-chain form has a single A-stationary kernel
@@ -1820,7 +1835,9 @@ sht_status sht_plan_autotune_ex(sht_plan* P, double alpha, double beta, int reps
 sht_status sht_plan_set_variant(sht_plan* P, int variant)
 {
   return guard([&] {
-    SHT_REQUIRE(P && variant >= 0 && variant <= 4, SHT_ERR_INVALID, "C-stationary variant must be 0..4");
+    SHT_REQUIRE(P && variant >= 0 && variant <= 5, SHT_ERR_INVALID, "C-stationary variant must be 0..5");
+    SHT_REQUIRE(variant != 5 || P->prec != SHT_PREC_FP64_EXACT, SHT_ERR_INVALID,
+                "the persistent variant needs the FP64, FP32 or BF16X3 form");
     if (variant != P->cs_variant) {
       drop_graph(P);
       P->cs_variant = variant;
@@ -1860,8 +1877,8 @@ sht_status sht_plan_get_fp32_staging(const sht_plan* P, int* on)
 
 const char* sht_cs_variant_name(int v)
 {
-  static const char* names[] = {"auto", "no_128row", "all_128row", "cost_order", "uniform_64x64"};
-  return v >= 0 && v <= 4 ? names[v] : "unknown";
+  static const char* names[] = {"auto", "no_128row", "all_128row", "cost_order", "uniform_64x64", "persistent"};
+  return v >= 0 && v <= 5 ? names[v] : "unknown";
 }
 
 sht_status sht_plan_tasks(sht_plan* P, const int64_t** ta, const int64_t** tb, const int64_t** tc,

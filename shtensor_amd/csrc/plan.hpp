@@ -37,7 +37,7 @@ struct TileRec {
   int32_t t0, t1;
   int64_t a0, b0;  // first task (t0): A and B block offsets and its K
   int32_t k0;
-  int32_t pad2;
+  int32_t kc;      // the tile's KClass (read by the persistent launch, whose tiles mix classes)
 };
 
 // Per C block: element offset, shape and task range (tasks sorted by K key).
@@ -68,6 +68,9 @@ struct ExecArgs {
   // A and B hold FP32 (the FP32 and split-BF16 forms read FP32 tensors);
   // C is always FP64
   int f32;
+  // persistent launches (the Persist classes): tiles per XCD queue (set by
+  // launch_exec)
+  int64_t qlen;
 };
 
 // kernel classes of the C-stationary launches
@@ -87,6 +90,9 @@ enum class KClass : int {
   Mfma64M, F32M,                     // FP64 / FP32, four waves per 64 x 32 tile (17-32 columns)
   F32Nq, F32Mq,                      // FP32, four waves per 128 x 16 / 128 x 32 tile (more
                                      // than 96 rows remain, large launches)
+  Persist64, Persist32, PersistBf16, // one persistent launch over every tile of the plan
+                                     // (FP64 / FP32 / split-BF16 four-wave classes, the
+                                     // class of each tile in its record)
   Count
 };
 
@@ -95,6 +101,11 @@ int kclass_cols(KClass k);
 int kclass_waves(KClass k);  // waves per workgroup (one tile per workgroup)
 
 void launch_exec(KClass k, const ExecArgs& a, hipStream_t s);
+
+// Grid of a persistent kernel f of `threads` threads per workgroup: the
+// workgroups the device keeps resident (a multiple of eight, one set per XCD
+// queue), at most one per work item (queried once per device and kernel).
+unsigned resident_grid(const void* f, int threads, int64_t nwork);
 
 // ---- A-stationary row-panel schedule --------------------------------------
 // A panel = the B blocks of one contracted key K (the partners of every A

@@ -496,9 +496,10 @@ def test_schedules_agree_on_uneven_block_rows(ctx, prec, scheds):
 @pytest.mark.parametrize("prec", [sht.PREC_FP64, sht.PREC_FP64_EXACT, sht.PREC_FP32, sht.PREC_BF16X3])
 def test_tile_variants_and_autotune_ex(ctx, prec):
     """Every C-stationary tile variant (automatic, without / with 128-row
-    tiles, cost order, uniform 64 x 64) gives the oracle's result (FP64 forms
-    bit for bit); the autotuner times every (schedule, variant) and the
-    variant it keeps gives the same result again."""
+    tiles, cost order, uniform 64 x 64, one persistent launch) gives the
+    oracle's result (FP64 forms bit for bit); the autotuner times every
+    (schedule, variant) and the variant it keeps gives the same result
+    again.  The fma chain has no persistent variant."""
     rng = np.random.default_rng(202)
     expr = "ik,kj->ij"
     bs_i = [5, 32, 33, 64, 97, 130, 161, 17]
@@ -522,16 +523,72 @@ def test_tile_variants_and_autotune_ex(ctx, prec):
         else:
             assert normwise(out, R.data) <= 1e-6
 
-    for v in range(5):
+    exact = prec == sht.PREC_FP64_EXACT
+    for v in range(5 if exact else 6):
         plan.set_variant(v)
         assert plan.get_variant() == v
         check()
+    if exact:
+        with pytest.raises(sht.ShtError, match="persistent variant"):
+            plan.set_variant(5)
+        assert plan.get_variant() == 4
     cands = plan.autotune_ex(1.0, 0.0, reps=1)
     names = {(s, v) for s, v, _ in cands}
     assert (0, "uniform_64x64") in names and (0, "auto") in names
+    assert ((0, "persistent") in names) == (not exact)
     assert all(ms > 0 for _, _, ms in cands)
     check()
     plan.close()
+
+
+@pytest.mark.parametrize("prec", [sht.PREC_FP64, sht.PREC_FP32, sht.PREC_BF16X3])
+def test_persistent_launch_many_tiles(ctx, prec):
+    """The persistent variant over more tiles than the device keeps
+    workgroups (every workgroup walks its XCD queue through several tiles):
+    4 096 C blocks of 32 x 32 (ik,kj->ij, L = 2048, p = 0.25) give the same C
+    bit for bit as the per-class launches of variant 0 (the same tile class,
+    32 x 32, in both), the FP64 form the oracle's; then a mix of widths
+    (ijk,km->ijm, uneven blocks from 5 to 40: 64 x 64, 64 x 32, 32 x 32 and
+    64 x 16 tiles in one launch) within the form's tolerance, FP64 bit for
+    bit."""
+    seed, L, b = 2026, 2048, 32
+    bs = [[b] * (L // b)] * 2
+    A = orc.BlockSparse(bs).reserve_ids(orc.gen_presence(seed, 0, [L // b] * 2, 0.25))
+    B = orc.BlockSparse(bs).reserve_ids(orc.gen_presence(seed, 1, [L // b] * 2, 0.25))
+    orc.gen_values(seed, 0, A, 0.1)
+    orc.gen_values(seed, 1, B, 0.1)
+    rng = np.random.default_rng(77)
+    cases = [("ik,kj->ij", A, B, orc.BlockSparse(bs))]
+    szs = {x: rng.choice([5, 9, 16, 17, 28, 33, 40], size=rng.integers(6, 10)).tolist() for x in "ijkm"}
+    A2 = random_bst(rng, [szs["i"], szs["j"], szs["k"]], 0.3)
+    B2 = random_bst(rng, [szs["k"], szs["m"]], 0.5)
+    cases.append(("ijk,km->ijm", A2, B2, orc.BlockSparse([szs["i"], szs["j"], szs["m"]])))
+    for n, (expr, a, bb, Cp) in enumerate(cases):
+        _, _, _, R = oracle_result(expr, a, bb, Cp)
+        dA, dB = to_device(ctx, a), to_device(ctx, bb)
+        outs = {}
+        for v in (0, 5):
+            dC = sht.Tensor(ctx, Cp.block_sizes)
+            plan = sht.Plan(ctx, expr, dA, dB, dC, prec)
+            plan.set_schedule(0)
+            plan.set_variant(v)
+            if v == 5:
+                assert plan.execute_timed(1.0, 0.0)[0][0].endswith("_persist")
+            print("persistent launch", expr, prec, "variant", v, flush=True)
+            plan.execute(1.0, 0.0)
+            sht.call("sht_device_synchronize")
+            outs[v] = dC.download()
+            assert np.array_equal(dC.get_indices(), R.indices)
+            plan.close()
+            dC.close()
+        if prec == sht.PREC_FP64:
+            assert np.array_equal(outs[5], R.data)
+        else:
+            assert normwise(outs[5], R.data) <= 1e-6
+        if n == 0:
+            assert np.array_equal(outs[5], outs[0])
+        dA.close()
+        dB.close()
 
 
 C1_FORMS = [(sht.PREC_FP64, (0, 1, 2, 3)), (sht.PREC_FP64_EXACT, (0, 1)), (sht.PREC_FP32, (0, 1, 2, 3)),

@@ -132,7 +132,7 @@ def test_fp32_operands_every_schedule_and_variant(ctx, form, case):
     orc.contract(expr, 0.5, A, B, 1.25, R)
     d64 = (to_device(ctx, A), to_device(ctx, B))
     d32 = (to_device32(ctx, A), to_device32(ctx, B))
-    runs = [(s, None) for s in scheds] + [(0, v) for v in range(5)]
+    runs = [(s, None) for s in scheds] + [(0, v) for v in range(6)]
     for sc, var in runs:
         print(form, expr, "schedule", sc, "variant", var, flush=True)
         o64, _ = run(ctx, expr, *d64, bsC, prec, sc, var, 0.5, 1.25, c0, cids)

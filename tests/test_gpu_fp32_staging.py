@@ -90,7 +90,7 @@ def test_staging_equals_unstaged_and_fp32_tensors(ctx, case, form):
     orc.contract(expr, 0.5, A, B, 1.25, R)
     d64 = (to_device(ctx, A), to_device(ctx, B))
     d32 = (to_device32(ctx, A), to_device32(ctx, B))
-    for var in range(5):
+    for var in range(6):
         print("fp32 staging", form, expr, "variant", var, flush=True)
         o0, i0 = run(ctx, expr, *d64, bsC, prec, False, var, 0.5, 1.25, c0, cids)
         o1, i1 = run(ctx, expr, *d64, bsC, prec, True, var, 0.5, 1.25, c0, cids)
