@@ -865,9 +865,10 @@ __global__ __launch_bounds__(64 * WPG) SHT_ASP_OCC void k_asp(AsArgs p)
 #ifdef SHT_AS_BF3_TU
 // ---------------------------------------------------------------------------
 // Split-BF16 for four waves over four 32-row chunks of one block row (ghost
-// units pad the workgroup).  Each 32-k step's panel fragments -- three bf16
-// planes of 16 columns per slot, 16 bytes per lane and plane -- are loaded
-// once per workgroup (wave w: slots w and w + 4) one step ahead and staged
+// units pad the workgroup).  Each 32-k step's panel fragments are read from
+// an FP32 panel (BtFormat::F32K; 32 bytes per lane and slot, two thirds of
+// the bytes of pre-split bf16 planes) once per workgroup (wave w: slots w and
+// w + 4) one step ahead, split into the three bf16 planes as they are staged
 // through two LDS stages, one barrier per step.  The raw A values of the next
 // step are loaded (unconditionally; ghosts read one fixed line) while the
 // current step's products run, and are masked (k >= K) and split when
@@ -885,9 +886,8 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
   const int lane = li + 16 * lk;
   const TE* A = static_cast<const TE*>(p.A) + sw.a_off;
   const int kp = sw.pitch;
-  const unsigned short* Bl =
-      static_cast<const unsigned short*>(p.B) + sw.b_off + static_cast<int64_t>(li) * kp + 8 * lk;
-  const int64_t plane = p.bt_total;
+  // FP32 panel (BtFormat::F32K): column-major, contracted indices contiguous
+  const float* Bl = static_cast<const float*>(p.B) + sw.b_off + static_cast<int64_t>(li) * kp + 8 * lk;
   const int64_t qstep = 16 * static_cast<int64_t>(kp);
   double* C = static_cast<double*>(p.C);
   const int K = sw.k, M = sw.m;
@@ -907,19 +907,29 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
   int64_t soff[NSW];
 #pragma unroll
   for (int x = 0; x < NSW; ++x) soff[x] = min(wave + 4 * x, NS - 1) * qstep;
-  v4u g[NSW][3];
+  // this wave's share of one step's panel: eight FP32 values (contracted
+  // indices 32st + 8lk .. +7 of column li) per slot, split into the three
+  // bf16 planes as they are written to LDS
+  v4f g[NSW][2];
   auto loadP = [&](int st) {
-    const unsigned short* b = Bl + kBfK * min(st, nsteps - 1);
+    const float* b = Bl + kBfK * min(st, nsteps - 1);
 #pragma unroll
-    for (int x = 0; x < NSW; ++x)
-#pragma unroll
-      for (int pl = 0; pl < 3; ++pl) g[x][pl] = *reinterpret_cast<const v4u*>(b + pl * plane + soff[x]);
+    for (int x = 0; x < NSW; ++x) {
+      g[x][0] = *reinterpret_cast<const v4f*>(b + soff[x]);
+      g[x][1] = *reinterpret_cast<const v4f*>(b + soff[x] + 4);
+    }
   };
   auto writeP = [&](int stg) {
 #pragma unroll
-    for (int x = 0; x < NSW; ++x)
-#pragma unroll
-      for (int pl = 0; pl < 3; ++pl) stage[stg][pl][wave + 4 * x][lane] = g[x][pl];
+    for (int x = 0; x < NSW; ++x) {
+      const float xf[8] = {g[x][0][0], g[x][0][1], g[x][0][2], g[x][0][3],
+                           g[x][1][0], g[x][1][1], g[x][1][2], g[x][1][3]};
+      v8s h, m, l;
+      split3x8(xf, h, m, l);
+      stage[stg][0][wave + 4 * x][lane] = __builtin_bit_cast(v4u, h);
+      stage[stg][1][wave + 4 * x][lane] = __builtin_bit_cast(v4u, m);
+      stage[stg][2][wave + 4 * x][lane] = __builtin_bit_cast(v4u, l);
+    }
   };
   // raw A of step st: rows rl2, rl2 + 1, contracted indices 32st + 8lk + e
   auto loadA = [&](int st, TE (&x)[R][8]) {
@@ -1112,6 +1122,24 @@ __global__ __launch_bounds__(64) void k_as_exact(AsArgs p)
 // ---------------------------------------------------------------------------
 // Panel conversion, once per execution (one workgroup per B block record).
 // ---------------------------------------------------------------------------
+// FP32 panel in the split-BF16 layout (BtFormat::F32K): column j of a record
+// at j * pitch, contracted indices contiguous, zero from k up to the pitch.
+template <class TE>
+__global__ __launch_bounds__(256) void k_bt_f32k(const BtRec* recs, int64_t nrec, const TE* B, float* BT)
+{
+  for (int64_t r = blockIdx.x; r < nrec; r += gridDim.x) {
+    const BtRec rec = recs[r];
+    const int kp = rec.pitch;
+    const int64_t tot = static_cast<int64_t>(kp) * rec.n;
+    const TE* b = B + rec.b_off;
+    float* o = BT + rec.bt_off;
+    for (int64_t x = threadIdx.x; x < tot; x += blockDim.x) {
+      const int j = static_cast<int>(x / kp), k = static_cast<int>(x - static_cast<int64_t>(j) * kp);
+      o[x] = k < rec.k ? static_cast<float>(b[k + static_cast<int64_t>(j) * rec.k]) : 0.f;
+    }
+  }
+}
+
 template <class T, class TE>
 __global__ __launch_bounds__(256) void k_bt(const BtRec* recs, int64_t nrec, const TE* B, T* BT)
 {
@@ -1305,6 +1333,12 @@ void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const void* B, bool 
         hipLaunchKernelGGL((k_bt<float, float>), dim3(g), dim3(256), 0, s, recs, nrec, b32, static_cast<float*>(BT));
       else
         hipLaunchKernelGGL((k_bt<float, double>), dim3(g), dim3(256), 0, s, recs, nrec, b64, static_cast<float*>(BT));
+      break;
+    case BtFormat::F32K:
+      if (b_f32)
+        hipLaunchKernelGGL(k_bt_f32k<float>, dim3(g), dim3(256), 0, s, recs, nrec, b32, static_cast<float*>(BT));
+      else
+        hipLaunchKernelGGL(k_bt_f32k<double>, dim3(g), dim3(256), 0, s, recs, nrec, b64, static_cast<float*>(BT));
       break;
     case BtFormat::BF16X3:
       if (b_f32)

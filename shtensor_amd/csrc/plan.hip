@@ -1283,6 +1283,8 @@ void build_as_schedule_impl(sht_plan* P)
   P->d_as_unitw = static_cast<AsUnit*>(keep(usw.data(), usw.size() * sizeof(AsUnit)));
   if (narrow) P->d_as_unit4 = static_cast<AsUnit*>(keep(us4.data(), us4.size() * sizeof(AsUnit)));
   P->d_bt_rec = static_cast<BtRec*>(keep(bt.data(), bt.size() * sizeof(BtRec)));
+  // (split-BF16: three bf16 planes, or one FP32 panel in the same layout for
+  // the grouped class, whichever schedule runs)
   const size_t bt_bytes = bf3 ? 3 * sizeof(uint16_t) : (P->prec == SHT_PREC_FP32 ? sizeof(float) : sizeof(double));
   P->d_bt = arena.allocate(std::max<int64_t>(bt_total, 1) * bt_bytes);
   // pad columns and rows of the panels stay zero: the conversion writes only
@@ -1339,7 +1341,8 @@ bool as_supported(const sht_plan* P)
 // (also used by the serving loop's merged A-stationary launches)
 BtFormat bt_format(const sht_plan* P)
 {
-  return P->prec == SHT_PREC_FP32 ? BtFormat::F32T : (P->prec == SHT_PREC_BF16X3 ? BtFormat::BF16X3 : BtFormat::F64T);
+  if (P->prec == SHT_PREC_BF16X3) return P->schedule == 2 ? BtFormat::F32K : BtFormat::BF16X3;
+  return P->prec == SHT_PREC_FP32 ? BtFormat::F32T : BtFormat::F64T;
 }
 
 AsClass as_class(const sht_plan* P)

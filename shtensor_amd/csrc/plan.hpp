@@ -145,7 +145,11 @@ struct BtRec {
   int32_t pitch;
   int32_t pad;
 };
-enum class BtFormat : int { F64T = 0, F32T = 1, BF16X3 = 2 };
+// Panel formats: FP64 / FP32 k-major rows (the FP64 and FP32 MFMA classes),
+// three bf16 planes with contracted indices contiguous per column (the
+// one-wave split-BF16 class), FP32 in that column layout (the grouped
+// split-BF16 class, which splits the panel as it stages it in LDS).
+enum class BtFormat : int { F64T = 0, F32T = 1, BF16X3 = 2, F32K = 3 };
 void launch_bt(BtFormat f, const BtRec* recs, int64_t nrec, const void* B, bool b_f32, void* BT, int64_t bt_total,
                hipStream_t s);
 struct AsUnit {
