@@ -904,9 +904,9 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
     for (int q = 0; q < NS; ++q) acc[t][q] = v4f{0.f, 0.f, 0.f, 0.f};
   // this wave's panel share (a share past the sweep's slots re-reads slot
   // NS - 1 into an unused stage slot)
-  int64_t soff[NSW];
+  int soff[NSW];  // (slot * 16 * pitch: below 2^31)
 #pragma unroll
-  for (int x = 0; x < NSW; ++x) soff[x] = min(wave + 4 * x, NS - 1) * qstep;
+  for (int x = 0; x < NSW; ++x) soff[x] = static_cast<int>(min(wave + 4 * x, NS - 1) * qstep);
   // this wave's share of one step's panel: eight FP32 values (contracted
   // indices 32st + 8lk .. +7 of column li) per slot, split into the three
   // bf16 planes as they are written to LDS
@@ -1019,7 +1019,7 @@ __device__ inline void sweep_bf3_lds(const AsArgs& p, const SweepU& sw, int r0, 
 // LDS-shared split-BF16 path).  Every wave runs the same sweep list, ghosts
 // included, so the workgroup barriers match.
 template <class TE>
-__global__ __launch_bounds__(256) SHT_ASP_OCC void k_asb(AsArgs p)
+__device__ __attribute__((always_inline)) inline void asb_body(const AsArgs& p)
 {
   constexpr int W = 4;
   const int wave = uni(static_cast<int>(threadIdx.x >> 6));
@@ -1058,6 +1058,23 @@ __global__ __launch_bounds__(256) SHT_ASP_OCC void k_asb(AsArgs p)
     nxt = to_uni(raw2);
     raw2 = p.sweeps[s0 + min(j + 3, nsw - 1)];
   }
+}
+
+// FP32 operands: held to 168 registers, three waves per SIMD (three
+// workgroups per CU with their 52 KB of LDS); the compiler then spills a few
+// sweep-level values, none inside the step loop, and C1 runs 33.5 -> 31.9 ms
+// against two waves per SIMD.  FP64 operands (more registers for the raw A
+// values) keep two waves per SIMD: at three they spill more and run slower
+// (40.2 -> 40.6 ms).
+template <class TE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_asb3(AsArgs p)
+{
+  asb_body<TE>(p);
+}
+template <class TE>
+__global__ __launch_bounds__(256) SHT_ASP_OCC void k_asb(AsArgs p)
+{
+  asb_body<TE>(p);
 }
 #endif  // SHT_AS_BF3_TU
 
@@ -1206,7 +1223,7 @@ void launch_as_bf3(AsClass k, const AsArgs& a, hipStream_t s)
       else hipLaunchKernelGGL((k_asp<2, 8, 2, 1, false, double>), g1, b1, 0, s, a);
       break;
     case AsClass::Bf16x3G:
-      if (a.f32) hipLaunchKernelGGL(k_asb<float>, gg, bg, 0, s, a);
+      if (a.f32) hipLaunchKernelGGL(k_asb3<float>, gg, bg, 0, s, a);
       else hipLaunchKernelGGL(k_asb<double>, gg, bg, 0, s, a);
       break;
     default: throw Error(SHT_ERR_INVALID, "not a split-BF16 class");
