@@ -240,8 +240,9 @@ const char* sht_cs_variant_name(int variant);
  * the reference, which has no reduced-precision form): with 1, each
  * C-stationary execution first converts A and B once to FP32 copies and the
  * tiles read those; with 0, the tiles read and round the FP64 values at every
- * use.  A new plan stages when its tiles read each operand element at least
- * ten times on average and the arena holds the copies (4 bytes per element).
+ * use.  A new plan that starts on the C-stationary schedule stages when its
+ * tiles read each operand element at least ten times on average and the
+ * arena holds the copies (4 bytes per element).
  * Both forms round every operand to FP32 first: results are bitwise the same
  * either way.  SHT_ERR_OOM when the arena cannot hold the copies.            */
 sht_status sht_plan_set_fp32_staging(sht_plan* p, int on);

@@ -602,9 +602,9 @@ bool make_stage32(sht_plan* P)
   return true;
 }
 
-// A new FP32 or split-BF16 plan over FP64 operands stages them in FP32 when
-// its C-stationary tiles read each operand element at least kStage32Reuse
-// times on average (the conversion then costs less than reading FP64 values
+// A new FP32 or split-BF16 plan over FP64 operands that starts on the
+// C-stationary schedule stages them in FP32 when its tiles read each operand
+// element at least kStage32Reuse times on average (the conversion then costs less than reading FP64 values
 // at every use; SHT_FP32_STAGING=0 or 1 overrides) and the arena holds the
 // copies.  Both forms round every operand to FP32 first, so results are
 // bitwise the same with or without the staging.
@@ -1589,12 +1589,12 @@ sht_status sht_plan_create(sht_ctx* ctx, const char* expr, const sht_tensor* A, 
       sht::build_tasks(P.get());
       const double t1 = sht::wall_ms();
       sht::build_schedule(P.get());
-      sht::build_stage32(P.get());
       P->ms_cstat = sht::wall_ms() - t1;
       P->ms_tasks = t1 - t0;
       sht::setup_staging(P.get());
       P->schedule = sht::default_schedule(P.get());
       if (P->schedule >= 1) sht::build_as_schedule(P.get());
+      else sht::build_stage32(P.get());  // (the copies serve the C-stationary tiles only)
     } catch (...) {
       P->release();
       throw;
